@@ -1,0 +1,201 @@
+"""MI355X-native GetHI density-to-sky path (CRIME's GetHI hot path on gfx950).
+
+Host-side mirror of the reference's operator sequence (src/main_gh.c:44-68)::
+
+    par = crime_amd.Params("param_GetHI.ini")      # read_run_params (src/io_gh.c:188-296)
+    gh = crime_amd.GetHI(par)                      # init_fftw + allocate_maps, on the device
+    gh.create_d_and_vr_fields()                    # src/fourier.c:375-438
+    gh.mk_T_maps()                                 # get_HI + mk_T_maps (src/grid_tools.c, src/pixelize.c)
+    par.write_maps(gh.maps())                      # write_maps (src/io_gh.c:109-131)
+
+Everything below the Python layer is C (host) and HIP (device) behind the C ABIs in ``include/``;
+this module only marshals arguments.  Errors surface as ``CrimeError`` with the library's message.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import (BIAS_CONST, BIAS_LINEAR, BIAS_REFERENCE, FP32, FP64, GRID_DENS, GRID_DZRSD, GRID_MASS,
+                   GRID_RVEL, GRID_VPOT, MODES_INJECTED, MODES_PHILOX, NU_TABLE, NU_UNIFORM, CrimeGhDesc)
+
+__all__ = ["Params", "GetHI", "CrimeError", "write_ini", "write_bbks_pk", "FP32", "FP64",
+           "GRID_DENS", "GRID_VPOT", "GRID_RVEL", "GRID_MASS", "GRID_DZRSD", "MODES_PHILOX",
+           "MODES_INJECTED", "NU_TABLE", "NU_UNIFORM", "BIAS_REFERENCE", "BIAS_CONST", "BIAS_LINEAR"]
+
+
+class CrimeError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"{_ffi.err_name(rc)} ({rc}): {msg}")
+        self.rc = rc
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def write_ini(path, **keys):
+    """Write a parameter file in the reference grammar ("key= value", src/io_gh.c:202-209)."""
+    with open(path, "w") as f:
+        f.write("# GetHI parameter file\n")
+        for k, v in keys.items():
+            f.write(f"{k}= {v}\n")
+    return path
+
+
+def write_bbks_pk(path, omega_m, omega_b, h, n_s, kmin=1e-5, kmax=1e2, n=2000):
+    """Tabulate A k^ns T_BBKS(k)^2 (Sugiyama shape parameter) in CAMB's two-column format."""
+    rc = _ffi.host_lib().gh_write_bbks_pk(os.fsencode(path), omega_m, omega_b, h, n_s, kmin, kmax, n)
+    if rc:
+        raise OSError(f"could not write {path}")
+    return path
+
+
+class Params:
+    """A GetHI run description parsed and set up by the host C library (libgethi_host.so)."""
+
+    def __init__(self, ini_path):
+        self._lib = _ffi.host_lib()
+        self._p = self._lib.gh_params_new()
+        if not self._p:
+            raise MemoryError("gh_params_new")
+        if self._lib.gh_params_load(self._p, os.fsencode(ini_path)):
+            msg = self._lib.gh_params_error(self._p).decode()
+            self.close()
+            raise ValueError(msg)
+
+    def get(self, name):
+        return self._lib.gh_params_get(self._p, name.encode())
+
+    def desc(self):
+        """crime_gh_desc pointing into this object's tables (valid while it lives)."""
+        d = CrimeGhDesc()
+        self._lib.gh_params_to_desc(self._p, C.byref(d))
+        return d
+
+    def write_maps(self, maps):
+        m = np.ascontiguousarray(maps, dtype=np.float32)
+        if self._lib.gh_write_maps(self._p, m.ctypes.data_as(C.POINTER(C.c_float))):
+            raise OSError("writing the FITS maps failed")
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._lib.gh_params_delete(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+
+class GetHI:
+    """One device handle of libcrime_gh.so (one HIP stream + resident fields and maps)."""
+
+    def __init__(self, params, **overrides):
+        self._lib = _ffi.gpu_lib()
+        d = params.desc() if isinstance(params, Params) else params
+        keep = []
+        for k, v in overrides.items():
+            if isinstance(v, np.ndarray):
+                arr = np.ascontiguousarray(v, dtype=np.float64)
+                keep.append(arr)
+                setattr(d, k, _dptr(arr))
+            else:
+                setattr(d, k, v)
+        self.desc = d
+        self.n = d.n_grid
+        self.n_nu = d.n_nu
+        self.n_side = d.n_side
+        self.n_sub = d.n_sub
+        self.npix = 12 * d.n_side * d.n_side
+        self.precision = d.precision
+        h = C.c_void_p()
+        rc = self._lib.crime_gh_create(C.byref(d), C.byref(h))
+        if rc:
+            raise CrimeError(rc, self._lib.crime_gh_last_error(None).decode())
+        self._h = h
+        self._params = params  # keeps the tables alive for the handle's life
+
+    def _check(self, rc):
+        if rc:
+            raise CrimeError(rc, self._lib.crime_gh_last_error(self._h).decode())
+
+    def _stage(self, fn):
+        st = _ffi.CrimeGhStats()
+        self._check(fn(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def create_d_and_vr_fields(self):
+        return self._stage(self._lib.crime_gh_create_d_and_vr_fields)
+
+    def mk_T_maps(self):
+        return self._stage(self._lib.crime_gh_mk_T_maps)
+
+    def run(self):
+        return self._stage(self._lib.crime_gh_run)
+
+    def maps(self, dtype=np.float32):
+        if np.dtype(dtype) == np.float64:
+            out = np.empty((self.n_nu, self.npix), np.float64)
+            self._check(self._lib.crime_gh_copy_maps_f64(self._h, _dptr(out)))
+        else:
+            out = np.empty((self.n_nu, self.npix), np.float32)
+            self._check(self._lib.crime_gh_copy_maps(self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def grid(self, which):
+        out = np.empty((self.n, self.n, self.n), np.float64)
+        self._check(self._lib.crime_gh_copy_grid(self._h, which, _dptr(out)))
+        return out
+
+    def set_grid(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.size == self.n ** 3
+        self._check(self._lib.crime_gh_set_grid(self._h, which, _dptr(a)))
+
+    def sigma2(self):
+        s2, m = C.c_double(), C.c_double()
+        self._check(self._lib.crime_gh_get_sigma2(self._h, C.byref(s2), C.byref(m)))
+        return s2.value, m.value
+
+    def set_sigma2(self, sigma2, mean=0.0):
+        self._check(self._lib.crime_gh_set_sigma2(self._h, sigma2, mean))
+
+    def sample_indices(self, cell0, ncell):
+        inu = np.empty((ncell, self.n_sub), np.int32)
+        ipix = np.empty((ncell, self.n_sub), np.int64)
+        self._check(self._lib.crime_gh_sample_indices(
+            self._h, cell0, ncell, inu.ctypes.data_as(C.POINTER(C.c_int32)),
+            ipix.ctypes.data_as(C.POINTER(C.c_int64))))
+        return inu, ipix
+
+    def device_maps(self):
+        p, nb = C.c_void_p(), C.c_size_t()
+        self._check(self._lib.crime_gh_device_maps(self._h, C.byref(p), C.byref(nb)))
+        return p.value, nb.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.crime_gh_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+def run_batch(handles):
+    """Run several handles through crime_gh_run_batch; returns one stats dict per handle."""
+    lib = _ffi.gpu_lib()
+    n = len(handles)
+    arr = (C.c_void_p * n)(*[h._h for h in handles])
+    st = (_ffi.CrimeGhStats * n)()
+    rc = lib.crime_gh_run_batch(arr, n, st)
+    if rc:
+        raise CrimeError(rc, lib.crime_gh_last_error(None).decode())
+    return [s.as_dict() for s in st]

@@ -1,0 +1,561 @@
+// C-ABI layer of libcrime_gh.so: handle life cycle, device arena, stage launches and the parity
+// accessors declared in include/crime_gh.h.  One handle = one HIP stream + its resident buffers.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "crime_gh.h"
+#include "gh_kernels.hpp"
+
+using namespace crime;
+
+struct crime_gh {
+  crime_gh_desc d;      // scalars of the descriptor (host array pointers are not kept)
+  int prec = 0, n = 0, n_nu = 0;
+  int64_t nside = 0, npix = 0;
+  hipStream_t stream = nullptr;
+  void *F = nullptr;
+  size_t ncell = 0;
+  void *maps = nullptr;
+  size_t nmaps = 0;
+  double *d_tab = nullptr;     // z_r2z | r_r2z | d_r2z | v_r2z
+  double *d_pk = nullptr;      // logk | pk
+  double *d_edges = nullptr;   // [n_nu+1]
+  double *d_prefac = nullptr;  // [n_nu]
+  double *d_sub = nullptr;     // [3*n_sub]
+  double *d_modes = nullptr;   // injected unit modes
+  void *d_tw = nullptr;        // twiddles exp(+2 pi i m/N)
+  double *d_partials = nullptr;
+  double *d_sig = nullptr;     // [sigma2, mean]
+  double *h_sig = nullptr;     // pinned mirror
+  int nparts = 0;
+  FillParams fill{};
+  MapParams map{};
+  double norm = 0;
+  bool fields_done = false;
+  hipEvent_t ev[8] = {};
+  std::string err;
+};
+
+static thread_local std::string g_create_err;
+
+static int set_error(crime_gh *h, int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf;
+  else g_create_err = buf;
+  return code;
+}
+
+#define HIP_OR_FAIL(h, call)                                                                        \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess)                                                                           \
+      return set_error(h, e_ == hipErrorOutOfMemory ? CRIME_GH_ENOMEM : CRIME_GH_EHIP, "%s: %s", #call, \
+                       hipGetErrorString(e_));                                                      \
+  } while (0)
+
+static bool grid_supported(int n)
+{
+  switch (n) {
+#define CRIME_CASE(NN) case NN:
+    CRIME_FOR_EACH_N(CRIME_CASE)
+#undef CRIME_CASE
+    return true;
+  default: return false;
+  }
+}
+
+static double r_of_z_host(const crime_gh_desc *d, double z)
+{
+  // r_of_z, src/cosmo.c:40-50
+  if (z <= 0) return 0;
+  if (z >= d->z_z2r[CRIME_GH_NZ - 1]) return d->r_z2r[CRIME_GH_NZ - 1];
+  int iz = (int)(z / CRIME_GH_DZ);
+  return d->r_z2r[iz] + (d->r_z2r[iz + 1] - d->r_z2r[iz]) * (z - d->z_z2r[iz]) / CRIME_GH_DZ;
+}
+
+static int validate(const crime_gh_desc *d)
+{
+  if (!d) return set_error(nullptr, CRIME_GH_EINVAL, "descriptor is NULL");
+  if (d->abi_version != CRIME_GH_ABI_VERSION)
+    return set_error(nullptr, CRIME_GH_EABI, "abi_version %u, library expects %u", d->abi_version,
+                     CRIME_GH_ABI_VERSION);
+  if (d->precision != CRIME_GH_FP32 && d->precision != CRIME_GH_FP64)
+    return set_error(nullptr, CRIME_GH_EINVAL, "precision %d", d->precision);
+  if (!grid_supported(d->n_grid))
+    return set_error(nullptr, CRIME_GH_EINVAL, "n_grid=%d has no FFT plan (2^a in [8,2048], 3*2^a in [12,1536])",
+                     d->n_grid);
+  if (d->n_side < 1 || d->n_side > (1 << 13)) return set_error(nullptr, CRIME_GH_EINVAL, "n_side=%ld", (long)d->n_side);
+  if (d->n_nu < 1) return set_error(nullptr, CRIME_GH_EINVAL, "n_nu=%d", d->n_nu);
+  if (d->nz != CRIME_GH_NZ) return set_error(nullptr, CRIME_GH_EINVAL, "nz=%d, tables must have %d entries", d->nz, CRIME_GH_NZ);
+  if (!d->z_z2r || !d->r_z2r || !d->z_r2z || !d->r_r2z || !d->d_r2z || !d->v_r2z)
+    return set_error(nullptr, CRIME_GH_EINVAL, "a radial table pointer is NULL");
+  if (!(d->idr > 0)) return set_error(nullptr, CRIME_GH_EINVAL, "idr must be positive");
+  if (!d->logk || !d->pk || d->numk < 2) return set_error(nullptr, CRIME_GH_EINVAL, "P(k) table missing");
+  if (!(d->l_box > 0)) return set_error(nullptr, CRIME_GH_EINVAL, "l_box must be positive");
+  if (d->n_sub < 1 || d->n_sub > 4096 || !d->sub_off)
+    return set_error(nullptr, CRIME_GH_EINVAL, "n_sub=%d / sub_off", d->n_sub);
+  if (d->nu_mode == CRIME_GH_NU_TABLE) {
+    if (!d->nu0 || !d->nuf) return set_error(nullptr, CRIME_GH_EINVAL, "channel edges missing");
+    for (int i = 0; i < d->n_nu; i++) {
+      if (!(d->nuf[i] > d->nu0[i])) return set_error(nullptr, CRIME_GH_EINVAL, "Frequency bins don't make sense");
+      if (i > 0 && d->nu0[i] != d->nuf[i - 1])
+        return set_error(nullptr, CRIME_GH_EINVAL, "channel %d does not start where channel %d ends", i, i - 1);
+    }
+  }
+  else if (d->nu_mode == CRIME_GH_NU_UNIFORM) {
+    if (!(d->nu_max > d->nu_min)) return set_error(nullptr, CRIME_GH_EINVAL, "nu_max must exceed nu_min");
+  }
+  else {
+    return set_error(nullptr, CRIME_GH_EINVAL, "nu_mode %d", d->nu_mode);
+  }
+  if (d->mode_source != CRIME_GH_MODES_PHILOX && d->mode_source != CRIME_GH_MODES_INJECTED)
+    return set_error(nullptr, CRIME_GH_EINVAL, "mode_source %d", d->mode_source);
+  if (d->mode_source == CRIME_GH_MODES_INJECTED && !d->unit_modes)
+    return set_error(nullptr, CRIME_GH_EINVAL, "unit_modes missing for injected modes");
+  if (d->bias_model < 0 || d->bias_model > 2) return set_error(nullptr, CRIME_GH_EINVAL, "bias_model %d", d->bias_model);
+  return 0;
+}
+
+template <typename T>
+static void twiddles(int n, std::vector<cx<T>> &tw)
+{
+  tw.resize(n);
+  for (int m = 0; m < n; m++) {
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+    tw[m].x = (T)cosl(a);
+    tw[m].y = (T)sinl(a);
+  }
+}
+
+static int free_all(crime_gh *h)
+{
+  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_sub,
+                  h->d_modes, h->d_tw, h->d_partials, h->d_sig};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->h_sig) (void)hipHostFree(h->h_sig);
+  for (auto &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  return 0;
+}
+
+extern "C" void crime_gh_destroy(crime_gh_t *h)
+{
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_all(h);
+  delete h;
+}
+
+extern "C" size_t crime_gh_desc_size(void) { return sizeof(crime_gh_desc); }
+
+extern "C" const char *crime_gh_last_error(const crime_gh_t *h)
+{
+  return h ? h->err.c_str() : g_create_err.c_str();
+}
+
+static int create_impl(const crime_gh_desc *d, crime_gh *h)
+{
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_error(h, CRIME_GH_ENODEV, "no HIP device visible");
+  if (d->device < 0 || d->device >= ndev) return set_error(h, CRIME_GH_ENODEV, "device %d of %d", d->device, ndev);
+  HIP_OR_FAIL(h, hipSetDevice(d->device));
+  hipDeviceProp_t prop;
+  HIP_OR_FAIL(h, hipGetDeviceProperties(&prop, d->device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(h, CRIME_GH_ENODEV, "device %d is %s; this library holds gfx950 code only", d->device,
+                     prop.gcnArchName);
+
+  h->d = *d;
+  h->prec = d->precision;
+  h->n = d->n_grid;
+  h->n_nu = d->n_nu;
+  h->nside = d->n_side;
+  h->npix = 12 * d->n_side * d->n_side;
+  h->ncell = (size_t)h->n * h->n * h->n;
+  h->nmaps = (size_t)h->npix * h->n_nu;
+  const size_t esz = h->prec ? sizeof(double) : sizeof(float);
+
+  HIP_OR_FAIL(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (auto &e : h->ev) HIP_OR_FAIL(h, hipEventCreate(&e));
+
+  HIP_OR_FAIL(h, hipMalloc(&h->F, h->ncell * 2 * esz));
+  HIP_OR_FAIL(h, hipMalloc(&h->maps, h->nmaps * esz));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_tab, 4 * CRIME_GH_NZ * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_pk, 2 * (size_t)d->numk * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_edges, (d->n_nu + 1) * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_prefac, d->n_nu * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_sub, 3 * (size_t)d->n_sub * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc(&h->d_tw, h->n * 2 * esz));
+  h->nparts = fields_partials_count(h->n, h->prec);
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_partials, 2 * (size_t)h->nparts * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_sig, 2 * sizeof(double)));
+  HIP_OR_FAIL(h, hipHostMalloc((void **)&h->h_sig, 2 * sizeof(double), hipHostMallocDefault));
+  size_t nhalf = (size_t)h->n * h->n * (h->n / 2 + 1);
+  if (d->mode_source == CRIME_GH_MODES_INJECTED) {
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_modes, 2 * nhalf * sizeof(double)));
+    HIP_OR_FAIL(h, hipMemcpy(h->d_modes, d->unit_modes, 2 * nhalf * sizeof(double), hipMemcpyHostToDevice));
+  }
+
+  // tables
+  const double *tabs[4] = {d->z_r2z, d->r_r2z, d->d_r2z, d->v_r2z};
+  for (int i = 0; i < 4; i++)
+    HIP_OR_FAIL(h, hipMemcpy(h->d_tab + (size_t)i * CRIME_GH_NZ, tabs[i], CRIME_GH_NZ * sizeof(double),
+                             hipMemcpyHostToDevice));
+  HIP_OR_FAIL(h, hipMemcpy(h->d_pk, d->logk, d->numk * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OR_FAIL(h, hipMemcpy(h->d_pk + d->numk, d->pk, d->numk * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OR_FAIL(h, hipMemcpy(h->d_sub, d->sub_off, 3 * (size_t)d->n_sub * sizeof(double), hipMemcpyHostToDevice));
+
+  // channel edges and mK factors (pixelize.c:308, 399-409)
+  std::vector<double> edges(d->n_nu + 1), prefac(d->n_nu);
+  double dnu_u = (d->nu_max - d->nu_min) / d->n_nu;
+  for (int i = 0; i < d->n_nu; i++) edges[i] = d->nu_mode == CRIME_GH_NU_TABLE ? d->nu0[i] : d->nu_min + dnu_u * i;
+  edges[d->n_nu] = d->nu_mode == CRIME_GH_NU_TABLE ? d->nuf[d->n_nu - 1] : d->nu_max;
+  double m2t = 90.057156 * d->omega_b * d->hhub * (double)h->npix / (4 * M_PI);
+  for (int i = 0; i < d->n_nu; i++) {
+    double dnu, nu;
+    if (d->nu_mode == CRIME_GH_NU_TABLE) {
+      dnu = d->nuf[i] - d->nu0[i];
+      nu = (d->nuf[i] + d->nu0[i]) * 0.5;
+    }
+    else {
+      dnu = (d->nu_max - d->nu_min) / d->n_nu;
+      nu = d->nu_min + (i + 0.5) * dnu;
+    }
+    double r = r_of_z_host(d, CRIME_GH_NU_21 / nu - 1);
+    prefac[i] = m2t / (r * r * dnu);
+  }
+  HIP_OR_FAIL(h, hipMemcpy(h->d_edges, edges.data(), edges.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OR_FAIL(h, hipMemcpy(h->d_prefac, prefac.data(), prefac.size() * sizeof(double), hipMemcpyHostToDevice));
+
+  if (h->prec) {
+    std::vector<cx<double>> tw;
+    twiddles(h->n, tw);
+    HIP_OR_FAIL(h, hipMemcpy(h->d_tw, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice));
+  }
+  else {
+    std::vector<cx<float>> tw;
+    twiddles(h->n, tw);
+    HIP_OR_FAIL(h, hipMemcpy(h->d_tw, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice));
+  }
+
+  // fill parameters (fourier.c:245-255)
+  FillParams &f = h->fill;
+  f.n = h->n;
+  f.dk = 2 * M_PI / d->l_box;
+  f.idk3 = 1. / (f.dk * f.dk * f.dk);
+  f.f0h0 = d->f0h0;
+  f.r2_smooth = d->r2_smooth;
+  f.do_smoothing = d->do_smoothing;
+  f.logk = h->d_pk;
+  f.pk = h->d_pk + d->numk;
+  f.numk = d->numk;
+  f.logkmin = d->logkmin;
+  f.logkmax = d->logkmax;
+  f.idlogk = d->idlogk;
+  f.n_s = d->n_s;
+  f.seed = d->seed;
+  f.mode_source = d->mode_source;
+  f.unit_modes = h->d_modes;
+  h->norm = pow(sqrt(2 * M_PI) / d->l_box, 3);  // fourier.c:403
+
+  // map parameters
+  MapParams &m = h->map;
+  m.n = h->n;
+  m.dx = d->l_box / h->n;
+  for (int i = 0; i < 3; i++) m.obs[i] = d->pos_obs[i];
+  m.tab.z = h->d_tab;
+  m.tab.r = h->d_tab + CRIME_GH_NZ;
+  m.tab.d = h->d_tab + 2 * CRIME_GH_NZ;
+  m.tab.v = h->d_tab + 3 * CRIME_GH_NZ;
+  m.tab.idr = d->idr;
+  m.tab.r_last = d->r_r2z[CRIME_GH_NZ - 1];
+  m.tab.z_last = d->z_r2z[CRIME_GH_NZ - 1];
+  m.tab.d_last = d->d_r2z[CRIME_GH_NZ - 1];
+  m.tab.v_last = d->v_r2z[CRIME_GH_NZ - 1];
+  m.bias_model = d->bias_model;
+  m.b0 = d->bias_b0;
+  m.b1 = d->bias_b1;
+  m.mass_prefac = m.dx * m.dx * m.dx;
+  m.rsd = d->rsd;
+  m.n_sub = d->n_sub;
+  m.sub_off = h->d_sub;
+  double hmax = 0;
+  for (int i = 0; i < d->n_sub; i++) {
+    const double *o = d->sub_off + 3 * i;
+    hmax = fmax(hmax, sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]));
+  }
+  m.h_sub = hmax * (1 + 1e-9) + 1e-6;
+  m.n_nu = d->n_nu;
+  m.nu_mode = d->nu_mode;
+  m.edges = h->d_edges;
+  m.nu_min = d->nu_min;
+  m.inv_dnu = d->n_nu / (d->nu_max - d->nu_min);
+  if (d->nu_mode == CRIME_GH_NU_TABLE) {
+    m.nu_lo = edges[0];
+    m.nu_hi = edges[d->n_nu];
+  }
+  else {
+    // (int) truncation bins frequencies down to nu_min - dnu into channel 0 (pixelize.c:369)
+    m.nu_lo = d->nu_min - dnu_u;
+    m.nu_hi = d->nu_max;
+  }
+  m.prefac = h->d_prefac;
+  m.nside = h->nside;
+  m.npix = h->npix;
+  m.sig = h->d_sig;
+  double zero2[2] = {0, 0};
+  HIP_OR_FAIL(h, hipMemcpy(h->d_sig, zero2, sizeof(zero2), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int crime_gh_create(const crime_gh_desc *desc, crime_gh_t **out)
+{
+  if (!out) return set_error(nullptr, CRIME_GH_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  crime_gh *h = new crime_gh();
+  rc = create_impl(desc, h);
+  if (rc) {
+    g_create_err = h->err;
+    free_all(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+static float ev_ms(hipEvent_t a, hipEvent_t b)
+{
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1;
+  return ms;
+}
+
+static int enqueue_fields(crime_gh *h, bool timed)
+{
+  hipError_t e;
+  if (h->prec)
+    e = launch_fields<double>(h->n, (cx<double> *)h->F, h->fill, (const cx<double> *)h->d_tw, h->norm,
+                              h->d_partials, h->d_sig, h->stream, timed ? h->ev : nullptr);
+  else
+    e = launch_fields<float>(h->n, (cx<float> *)h->F, h->fill, (const cx<float> *)h->d_tw, h->norm,
+                             h->d_partials, h->d_sig, h->stream, timed ? h->ev : nullptr);
+  if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "field kernels: %s", hipGetErrorString(e));
+  return 0;
+}
+
+static int enqueue_maps(crime_gh *h, bool timed)
+{
+  hipError_t e;
+  if (h->prec)
+    e = launch_maps<double>((const cx<double> *)h->F, (double *)h->maps, h->nmaps, h->map, h->stream,
+                            timed ? h->ev + 5 : nullptr);
+  else
+    e = launch_maps<float>((const cx<float> *)h->F, (float *)h->maps, h->nmaps, h->map, h->stream,
+                           timed ? h->ev + 5 : nullptr);
+  if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "map kernels: %s", hipGetErrorString(e));
+  return 0;
+}
+
+static int finish(crime_gh *h, crime_gh_stats *st, bool fields, bool maps)
+{
+  HIP_OR_FAIL(h, hipMemcpyAsync(h->h_sig, h->d_sig, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+  if (st) {
+    if (fields) {
+      st->ms_fill_fft_z = ev_ms(h->ev[0], h->ev[1]);
+      st->ms_fft_y = ev_ms(h->ev[1], h->ev[2]);
+      st->ms_fft_x = ev_ms(h->ev[2], h->ev[3]);
+      st->ms_sigma2 = ev_ms(h->ev[3], h->ev[4]);
+    }
+    if (maps) {
+      st->ms_maps_clear = ev_ms(h->ev[5], h->ev[6]);
+      st->ms_pixelize = ev_ms(h->ev[6], h->ev[7]);
+    }
+    st->ms_total = ev_ms(fields ? h->ev[0] : h->ev[5], maps ? h->ev[7] : h->ev[4]);
+    st->sigma2_gauss = h->h_sig[0];
+    st->mean_gauss = h->h_sig[1];
+  }
+  return 0;
+}
+
+extern "C" int crime_gh_create_d_and_vr_fields(crime_gh_t *h, crime_gh_stats *st)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  int rc = enqueue_fields(h, true);
+  if (rc) return rc;
+  rc = finish(h, st, true, false);
+  if (rc) return rc;
+  h->fields_done = true;
+  return 0;
+}
+
+extern "C" int crime_gh_mk_T_maps(crime_gh_t *h, crime_gh_stats *st)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (!h->fields_done) return set_error(h, CRIME_GH_ESTATE, "mk_T_maps before create_d_and_vr_fields");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  int rc = enqueue_maps(h, true);
+  if (rc) return rc;
+  return finish(h, st, false, true);
+}
+
+extern "C" int crime_gh_run(crime_gh_t *h, crime_gh_stats *st)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  int rc = enqueue_fields(h, true);
+  if (rc) return rc;
+  rc = enqueue_maps(h, true);
+  if (rc) return rc;
+  rc = finish(h, st, true, true);
+  if (rc) return rc;
+  h->fields_done = true;
+  return 0;
+}
+
+extern "C" int crime_gh_copy_maps(crime_gh_t *h, float *out)
+{
+  if (!h || !out) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  if (!h->prec) {
+    HIP_OR_FAIL(h, hipMemcpy(out, h->maps, h->nmaps * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<double> tmp(h->nmaps);
+  HIP_OR_FAIL(h, hipMemcpy(tmp.data(), h->maps, h->nmaps * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < h->nmaps; i++) out[i] = (float)tmp[i];
+  return 0;
+}
+
+extern "C" int crime_gh_copy_maps_f64(crime_gh_t *h, double *out)
+{
+  if (!h || !out) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  if (h->prec) {
+    HIP_OR_FAIL(h, hipMemcpy(out, h->maps, h->nmaps * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<float> tmp(h->nmaps);
+  HIP_OR_FAIL(h, hipMemcpy(tmp.data(), h->maps, h->nmaps * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < h->nmaps; i++) out[i] = tmp[i];
+  return 0;
+}
+
+extern "C" int crime_gh_get_sigma2(crime_gh_t *h, double *sigma2, double *mean)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  double v[2];
+  HIP_OR_FAIL(h, hipMemcpy(v, h->d_sig, sizeof(v), hipMemcpyDeviceToHost));
+  if (sigma2) *sigma2 = v[0];
+  if (mean) *mean = v[1];
+  return 0;
+}
+
+extern "C" int crime_gh_set_sigma2(crime_gh_t *h, double sigma2, double mean)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  double v[2] = {sigma2, mean};
+  HIP_OR_FAIL(h, hipMemcpy(h->d_sig, v, sizeof(v), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int crime_gh_copy_grid(crime_gh_t *h, int which, double *dst)
+{
+  if (!h || !dst) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (which < CRIME_GH_GRID_DENS || which > CRIME_GH_GRID_DZRSD) return set_error(h, CRIME_GH_EINVAL, "grid %d", which);
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  double *tmp = nullptr;
+  HIP_OR_FAIL(h, hipMalloc((void **)&tmp, h->ncell * sizeof(double)));
+  hipError_t e;
+  if (which <= CRIME_GH_GRID_VPOT)
+    e = h->prec ? launch_grid_get<double>((const cx<double> *)h->F, h->ncell, which, tmp, h->stream)
+                : launch_grid_get<float>((const cx<float> *)h->F, h->ncell, which, tmp, h->stream);
+  else
+    e = h->prec ? launch_cell_field<double>((const cx<double> *)h->F, h->map, which, tmp, h->stream)
+                : launch_cell_field<float>((const cx<float> *)h->F, h->map, which, tmp, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(dst, tmp, h->ncell * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "copy_grid: %s", hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" int crime_gh_set_grid(crime_gh_t *h, int which, const double *src)
+{
+  if (!h || !src) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (which != CRIME_GH_GRID_DENS && which != CRIME_GH_GRID_VPOT)
+    return set_error(h, CRIME_GH_EINVAL, "only DENS and VPOT can be set");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  double *tmp = nullptr;
+  HIP_OR_FAIL(h, hipMalloc((void **)&tmp, h->ncell * sizeof(double)));
+  hipError_t e = hipMemcpy(tmp, src, h->ncell * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = h->prec ? launch_grid_set<double>((cx<double> *)h->F, h->ncell, which, tmp, h->stream)
+                : launch_grid_set<float>((cx<float> *)h->F, h->ncell, which, tmp, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "set_grid: %s", hipGetErrorString(e));
+  h->fields_done = true;
+  return 0;
+}
+
+extern "C" int crime_gh_sample_indices(crime_gh_t *h, int64_t cell0, int64_t ncell, int32_t *inu, int64_t *ipix)
+{
+  if (!h || !inu || !ipix) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (cell0 < 0 || ncell < 1 || (size_t)(cell0 + ncell) > h->ncell)
+    return set_error(h, CRIME_GH_EINVAL, "cell range [%lld,%lld) outside the grid", (long long)cell0,
+                     (long long)(cell0 + ncell));
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  size_t ns = (size_t)ncell * h->map.n_sub;
+  int32_t *d_inu = nullptr;
+  int64_t *d_pix = nullptr;
+  HIP_OR_FAIL(h, hipMalloc((void **)&d_inu, ns * sizeof(int32_t)));
+  hipError_t e = hipMalloc((void **)&d_pix, ns * sizeof(int64_t));
+  if (e == hipSuccess)
+    e = h->prec ? launch_sample_indices<double>((const cx<double> *)h->F, h->map, cell0, ncell, d_inu, d_pix, h->stream)
+                : launch_sample_indices<float>((const cx<float> *)h->F, h->map, cell0, ncell, d_inu, d_pix, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(inu, d_inu, ns * sizeof(int32_t), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ipix, d_pix, ns * sizeof(int64_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d_inu);
+  if (d_pix) (void)hipFree(d_pix);
+  if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "sample_indices: %s", hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (dptr) *dptr = h->maps;
+  if (bytes) *bytes = h->nmaps * (h->prec ? sizeof(double) : sizeof(float));
+  return 0;
+}
+
+extern "C" int crime_gh_run_batch(crime_gh_t **hs, int n, crime_gh_stats *st)
+{
+  if (!hs || n < 1) return set_error(nullptr, CRIME_GH_EINVAL, "empty batch");
+  for (int i = 0; i < n; i++) {
+    int rc = crime_gh_run(hs[i], st ? st + i : nullptr);
+    if (rc) return rc;
+  }
+  return 0;
+}

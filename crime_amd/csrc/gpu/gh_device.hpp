@@ -1,0 +1,188 @@
+// Device-side arithmetic of the GetHI density-to-sky path for gfx950.
+//
+// Everything that decides an integer (table index, channel, HEALPix pixel) is FP64 with FMA
+// contraction disabled, so that it performs the same IEEE operations in the same order as the
+// host-side restatement of the reference (src/cosmo.c:52-86, src/pixelize.c:181-208, 343-381).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace crime {
+
+template <typename T>
+struct __align__(2 * sizeof(T)) cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b)
+{
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by +i
+template <typename T>
+__device__ __forceinline__ cx<T> cmuli(cx<T> a) { return {-a.y, a.x}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.x, -a.y}; }
+
+// ---------------------------------------------------------------------------------------------
+// Philox-4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11): counter-based, so mode k can be drawn by
+// any thread in any order and the mode array does not depend on the launch geometry.
+// ---------------------------------------------------------------------------------------------
+struct u32x4 {
+  uint32_t v[4];
+};
+
+__host__ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)
+{
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)M0 * c.v[0];
+    uint64_t p1 = (uint64_t)M1 * c.v[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    u32x4 o;
+    o.v[0] = hi1 ^ c.v[1] ^ k0;
+    o.v[1] = lo1;
+    o.v[2] = hi0 ^ c.v[3] ^ k1;
+    o.v[3] = lo0;
+    c = o;
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
+// 53-bit uniform in [0,1) from two 32-bit words
+__host__ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo)
+{
+  uint64_t m = ((uint64_t)hi << 21) ^ (uint64_t)(lo >> 11);
+  return (double)(m & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// uniform-r table lookups (src/cosmo.c:52-86)
+// ---------------------------------------------------------------------------------------------
+struct RTables {
+  const double *z;    // z_arr_r2z
+  const double *r;    // r_arr_r2z
+  const double *d;    // growth_d_arr
+  const double *v;    // growth_v_arr
+  double idr;
+  double r_last;      // r_arr_r2z[NZ-1]
+  double z_last, d_last, v_last;
+};
+
+__device__ __forceinline__ double tab_lerp(const double *y, const double *rr, double idr, double r,
+                                           double r_last, double y_last, double y_at0)
+{
+#pragma clang fp contract(off)
+  if (r <= 0) return y_at0;
+  if (r >= r_last) return y_last;
+  int ir = (int)(r * idr);
+  if (ir > 5001 - 2) ir = 5001 - 2;  // r within one ulp of the last node: stay inside the table
+  double y0 = y[ir], y1 = y[ir + 1];
+  return y0 + (y1 - y0) * (r - rr[ir]) * idr;
+}
+
+__device__ __forceinline__ double z_of_r(const RTables &t, double r)
+{
+  return tab_lerp(t.z, t.r, t.idr, r, t.r_last, t.z_last, 0.0);
+}
+__device__ __forceinline__ double dgrowth_of_r(const RTables &t, double r)
+{
+  return tab_lerp(t.d, t.r, t.idr, r, t.r_last, t.d_last, 1.0);
+}
+__device__ __forceinline__ double vgrowth_of_r(const RTables &t, double r)
+{
+  return tab_lerp(t.v, t.r, t.idr, r, t.r_last, t.v_last, 1.0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// channel of a frequency.  TABLE mode: the unique i with nu0[i] <= nu < nuf[i] (edges[i] = nu0[i],
+// edges[n] = nuf[n-1]); get_inu's walk (src/pixelize.c:181-208) converges to this index from any
+// start because the bins are contiguous.  UNIFORM mode: (int)(inv_dnu*(nu-nu_min)) (:369).
+// Returns -1 when no channel receives the sample.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int channel_of(double nu, int n_nu, int nu_mode, const double *edges,
+                                          double nu_min, double inv_dnu)
+{
+#pragma clang fp contract(off)
+  if (nu_mode == 1) {
+    // (int)t lands in [0, n_nu) exactly when -1 < t < n_nu; testing t first keeps the cast defined
+    double t = inv_dnu * (nu - nu_min);
+    if (!(t > -1.0 && t < (double)n_nu)) return -1;
+    return (int)t;
+  }
+  if (!(nu >= edges[0]) || !(nu < edges[n_nu])) return -1;
+  int lo = 0, hi = n_nu;  // edges[lo] <= nu < edges[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (nu >= edges[mid]) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------------------------
+// RING pixel of a (non-normalised) direction, Gorski et al. 2005 (ApJ 622:759) section 4; vlen is
+// the vector's length sqrt(x*x+y*y+z*z).  Near the poles (|cos theta| > 0.99) sin theta is taken
+// from sqrt(x*x+y*y)/vlen to avoid the cancellation in 1-|cos theta|.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int64_t imod64(int64_t v, int64_t m)
+{
+  int64_t r = v % m;
+  return r < 0 ? r + m : r;
+}
+
+__device__ __forceinline__ int64_t vec2pix_ring(int64_t nside, double x, double y, double z, double vlen)
+{
+#pragma clang fp contract(off)
+  const double twopi = 6.283185307179586476925286766559;
+  const double inv_halfpi = 0.636619772367581343075535053490057;
+  const double twothird = 2.0 / 3.0;
+  double cth = z / vlen;
+  double za = fabs(cth);
+  double phi = atan2(y, x);
+  double ph = phi;
+  if (ph < 0) {
+    ph = ph + twopi;
+    if (ph == twopi) ph = 0;
+  }
+  double tt = ph * inv_halfpi;  // [0,4)
+  double dn = (double)nside;
+  if (za <= twothird) {
+    double temp1 = dn * (0.5 + tt);
+    double temp2 = dn * cth * 0.75;
+    int64_t jp = (int64_t)(temp1 - temp2);
+    int64_t jm = (int64_t)(temp1 + temp2);
+    int64_t ir = nside + 1 + jp - jm;
+    int64_t kshift = 1 - (ir & 1);
+    int64_t ip = (jp + jm - nside + kshift + 1) / 2;
+    ip = imod64(ip, 4 * nside);
+    return nside * (nside - 1) * 2 + (ir - 1) * 4 * nside + ip;
+  }
+  double tp = tt - (double)(int)tt;
+  double tmp;
+  if (za > 0.99) {
+    double sth = sqrt(x * x + y * y) / vlen;
+    tmp = dn * sth / sqrt((1.0 + za) / 3.0);
+  }
+  else {
+    tmp = dn * sqrt(3.0 * (1.0 - za));
+  }
+  int64_t jp = (int64_t)(tp * tmp);
+  int64_t jm = (int64_t)((1.0 - tp) * tmp);
+  int64_t ir = jp + jm + 1;
+  int64_t ip = (int64_t)(tt * (double)ir);
+  ip = imod64(ip, 4 * ir);
+  if (cth > 0) return 2 * ir * (ir - 1) + ip;
+  return 12 * nside * nside - 2 * ir * (ir + 1) + ip;
+}
+}  // namespace crime

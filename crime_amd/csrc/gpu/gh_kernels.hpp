@@ -1,0 +1,114 @@
+// Shared declarations of the gfx950 kernels of the GetHI density-to-sky path: launch shapes,
+// kernel parameter blocks and the host-side launchers called by the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gh_fft.hpp"
+
+namespace crime {
+
+// grid sizes with a compiled FFT plan: 2^a (8..2048) and 3*2^a (12..1536)
+#define CRIME_FOR_EACH_N(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) \
+  X(1536) X(2048)
+
+constexpr uint32_t PHILOX_KEY1 = 0x47484931u;  // second Philox key word ("GHI1")
+
+constexpr int cpow2_div(int n)
+{
+  int p = 1;
+  while (n % (2 * p) == 0) p *= 2;
+  return p;
+}
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// max elements per thread kept in VGPRs: 64 complex float or 32 complex double
+template <typename T>
+constexpr int emax() { return sizeof(T) == 4 ? 64 : 32; }
+
+// strided (kz, ky) passes: TX adjacent columns per workgroup, TC threads per column
+template <typename T, int N>
+struct ColCfg {
+  static constexpr int TC = plan_tc(N, cmax(16, N / emax<T>()));
+  static constexpr int CB = N * (int)sizeof(cx<T>);
+  static constexpr int TXMAX = cmin(16, cpow2_div(N));
+  static constexpr int tx_for(int budget)
+  {
+    int tx = TXMAX;
+    while (tx > 1 && tx * CB > budget) tx /= 2;
+    return tx;
+  }
+  static constexpr int TX = (tx_for(65536) >= 8 || tx_for(65536) == TXMAX) ? tx_for(65536) : tx_for(131072);
+  static constexpr int NT = TX * TC;
+};
+
+// contiguous kx pass: TY rows per workgroup, TC threads per row
+template <typename T, int N>
+struct RowCfg {
+  static constexpr int TC = plan_tc(N, cmax(64, N / emax<T>()));
+  static constexpr int RB = N * (int)sizeof(cx<T>);
+  static constexpr int ty_for(int budget)
+  {
+    int ty = cmin(cmax(256 / TC, 1), cpow2_div(N * N));
+    while (ty > 1 && ty * RB > budget) ty /= 2;
+    return ty;
+  }
+  static constexpr int TY = ty_for(65536) * RB <= 65536 ? ty_for(65536) : ty_for(131072);
+  static constexpr int NT = TY * TC;
+};
+
+struct FillParams {
+  int n;
+  double dk, idk3, f0h0, r2_smooth;
+  int do_smoothing;
+  const double *logk, *pk;
+  int numk;
+  double logkmin, logkmax, idlogk, n_s;
+  uint32_t seed;
+  int mode_source;
+  const double *unit_modes;
+};
+
+struct MapParams {
+  int n;
+  double dx, obs[3];
+  RTables tab;
+  int bias_model;
+  double b0, b1;
+  double mass_prefac;      // dx^3 (grid_tools.c:122)
+  int rsd;
+  int n_sub;
+  const double *sub_off;   // [3*n_sub]
+  double h_sub;            // bound on |offset| (with rounding margin)
+  int n_nu, nu_mode;
+  const double *edges;     // [n_nu+1]
+  double nu_min, inv_dnu;
+  double nu_lo, nu_hi;     // a sample can be binned only if nu_lo <= nu < nu_hi
+  const double *prefac;    // [n_nu] mK factor per channel (pixelize.c:399-409)
+  int64_t nside, npix;
+  const double *sig;       // device [sigma2, mean]
+};
+
+template <typename T>
+hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
+                         double *partials, double *sig, hipStream_t s, hipEvent_t *ev);
+int fields_partials_count(int n, int prec);
+
+template <typename T>
+hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m, hipStream_t s,
+                       hipEvent_t *ev);
+template <typename T>
+hipError_t launch_cell_field(const cx<T> *F, const MapParams &m, int which, double *out, hipStream_t s);
+template <typename T>
+hipError_t launch_sample_indices(const cx<T> *F, const MapParams &m, int64_t cell0, int64_t ncell,
+                                 int32_t *inu, int64_t *ipix, hipStream_t s);
+template <typename T>
+hipError_t launch_grid_get(const cx<T> *F, size_t ncell, int which, double *out, hipStream_t s);
+template <typename T>
+hipError_t launch_grid_set(cx<T> *F, size_t ncell, int which, const double *in, hipStream_t s);
+
+}  // namespace crime

@@ -1,0 +1,246 @@
+// Field stage of the GetHI density-to-sky path on gfx950: create_d_and_vr_fields
+// (src/fourier.c:234-305, 375-438) as three HBM passes over one complex N^3 grid.
+//
+// delta and the velocity potential phi are both real, so the two inverse real FFTs of the
+// reference (fourier.c:391-392) are done as ONE complex inverse FFT of
+//     F(k) = delta_k + i phi_k,   phi_k = delta_k f0 H0 / k^2            (fourier.c:297-298)
+// over the Hermitian completion of the half spectrum: the output is delta in the real part and phi
+// in the imaginary part of each cell.  Bytes moved equal those of two r2c-layout transforms, the
+// map stage then reads (delta, phi) of a cell in one 8-byte load.  The kx = 0 and kx = N/2 planes,
+// which the reference fills without Hermitian symmetry (fourier.c:275-299), enter through their
+// Hermitian part (X(k) + X*(-k))/2, which is what a c2r transform uses of them.
+//
+//   pass 1  k_fill_z : generate F on the fly (P(k) amplitude x unit mode) and transform along kz
+//   pass 2  k_fft_y  : transform along ky
+//   pass 3  k_fft_x  : transform along kx (contiguous), scale by (sqrt(2 pi)/L)^3 (fourier.c:403),
+//                      FP64 partial sums of delta and delta^2 (compute_sigma_dens, fourier.c:24-76)
+//   k_sigma_finalize : fixed-order FP64 reduction of the partials -> sigma^2, <delta>
+#include "gh_kernels.hpp"
+
+namespace crime {
+
+// ---------------------------------------------------------------------------------------------
+// mode fill
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double pk_linear0(const FillParams &f, double lgk)
+{
+#pragma clang fp contract(off)
+  // src/cosmo.c:153-170; the last half-open bin uses the k^-3 tail instead of reading past the table
+  int ik = (int)((lgk - f.logkmin) * f.idlogk);
+  if (ik < 0) return f.pk[0] * pow(10.0, f.n_s * (lgk - f.logkmin));
+  if (ik < f.numk - 1) return f.pk[ik] + (lgk - f.logk[ik]) * (f.pk[ik + 1] - f.pk[ik]) * f.idlogk;
+  return f.pk[f.numk - 1] * pow(10.0, -3 * (lgk - f.logkmax));
+}
+
+// unit-variance complex mode at half-spectrum position (kx in [0,N/2], ky, kz)
+__device__ __forceinline__ cx<double> unit_mode(const FillParams &f, int kx, int ky, int kz)
+{
+  size_t idx = (size_t)kx + (size_t)(f.n / 2 + 1) * ((size_t)ky + (size_t)f.n * (size_t)kz);
+  if (f.mode_source == 1) return {f.unit_modes[2 * idx], f.unit_modes[2 * idx + 1]};
+  u32x4 c;
+  c.v[0] = (uint32_t)idx;
+  c.v[1] = (uint32_t)(idx >> 32);
+  c.v[2] = 0;
+  c.v[3] = 0;
+  u32x4 r = philox4x32_10(c, f.seed, PHILOX_KEY1);
+  double u1 = u53(r.v[0], r.v[1]);
+  double u2 = u53(r.v[2], r.v[3]);
+  // rng_delta_gauss (src/common.c:154-164): phase 2 pi u1, modulus sqrt(-ln(1-u2))
+  double mod = sqrt(-log(1.0 - u2));
+  double s, co;
+  sincos(2.0 * M_PI * u1, &s, &co);
+  return {mod * co, mod * s};
+}
+
+// F = delta + i phi at full-grid position (kx, ky, kz), kx in [0, N)
+template <typename T>
+__device__ __forceinline__ cx<T> fill_element(const FillParams &f, int kx, int ky, int kz)
+{
+  const int n = f.n, h = n / 2;
+  if ((kx | ky | kz) == 0) return {(T)0, (T)0};
+  // |k| components with the reference's +Nyquist convention (fourier.c:264-283); only squares matter
+  int ax = kx <= h ? kx : n - kx;
+  int ay = ky <= h ? ky : n - ky;
+  int az = kz <= h ? kz : n - kz;
+  double kxd = ax * f.dk, kyd = ay * f.dk, kzd = az * f.dk;
+  double k2;
+  {
+#pragma clang fp contract(off)
+    k2 = kxd * kxd + kyd * kyd + kzd * kzd;
+  }
+  double sigma2 = pk_linear0(f, 0.5 * log10(k2)) * f.idk3;
+  if (f.do_smoothing) sigma2 *= exp(-f.r2_smooth * k2);
+  double amp = sqrt(sigma2);
+  double psi = f.f0h0 / k2;
+  int myn = (n - ky) % n, mzn = (n - kz) % n;  // -k in y and z
+  cx<double> g;
+  if (kx > 0 && kx < h) {
+    g = unit_mode(f, kx, ky, kz);
+  }
+  else if (kx > h) {
+    cx<double> m = unit_mode(f, n - kx, myn, mzn);
+    g = {m.x, -m.y};
+  }
+  else {  // kx == 0 or kx == N/2: Hermitian part of the plane
+    cx<double> a = unit_mode(f, kx, ky, kz), b = unit_mode(f, kx, myn, mzn);
+    g = {0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+  }
+  double dx_ = amp * g.x, dy_ = amp * g.y;
+  return {(T)(dx_ - psi * dy_), (T)(dy_ + psi * dx_)};
+}
+
+// ---------------------------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_fill_z(cx<T> *__restrict__ F, FillParams f,
+                                                             const cx<T> *__restrict__ tw)
+{
+  using C = ColCfg<T, N>;
+  __shared__ cx<T> lds[N * C::TX];
+  const int c = threadIdx.x % C::TX, t = threadIdx.x / C::TX;
+  const int kx = blockIdx.x * C::TX + c, ky = blockIdx.y;
+  const size_t n2 = (size_t)N * N;
+  cx<T> *col = F + (size_t)kx + (size_t)N * ky;
+  // the generator is too large to unroll into the register tile: stage the modes in LDS first
+  for (int e = t; e < N; e += C::TC) lds[e * C::TX + c] = fill_element<T>(f, kx, ky, e);
+  __syncthreads();
+  auto load = [&](int e) { return lds[e * C::TX + c]; };
+  auto store = [&](int e, cx<T> v) { col[(size_t)e * n2] = v; };
+  auto slot = [&](int e) -> cx<T> & { return lds[e * C::TX + c]; };
+  Stockham<T, N, C::TC>::template run<true>(t, load, store, slot, tw);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_fft_y(cx<T> *__restrict__ F,
+                                                            const cx<T> *__restrict__ tw)
+{
+  using C = ColCfg<T, N>;
+  __shared__ cx<T> lds[N * C::TX];
+  const int c = threadIdx.x % C::TX, t = threadIdx.x / C::TX;
+  const int kx = blockIdx.x * C::TX + c, kz = blockIdx.y;
+  cx<T> *col = F + (size_t)kx + (size_t)N * N * kz;
+  auto load = [&](int e) { return col[(size_t)e * N]; };
+  auto store = [&](int e, cx<T> v) { col[(size_t)e * N] = v; };
+  auto slot = [&](int e) -> cx<T> & { return lds[e * C::TX + c]; };
+  Stockham<T, N, C::TC>::run(t, load, store, slot, tw);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_fft_x(cx<T> *__restrict__ F,
+                                                            const cx<T> *__restrict__ tw,
+                                                            double norm, double *__restrict__ partials)
+{
+  using C = RowCfg<T, N>;
+  __shared__ cx<T> lds[N * C::TY];
+  __shared__ double red[2 * C::NT];
+  const int t = threadIdx.x % C::TC, b = threadIdx.x / C::TC;
+  const size_t row = (size_t)blockIdx.x * C::TY + b;
+  cx<T> *line = F + row * N;
+  double s1 = 0, s2 = 0;
+  auto load = [&](int e) { return line[e]; };
+  auto store = [&](int e, cx<T> v) {
+    cx<T> o = {(T)((double)v.x * norm), (T)((double)v.y * norm)};
+    double d = (double)o.x;
+    s1 += d;
+    s2 += d * d;
+    line[e] = o;
+  };
+  auto slot = [&](int e) -> cx<T> & { return lds[b * N + e]; };
+  Stockham<T, N, C::TC>::run(t, load, store, slot, tw);
+  red[threadIdx.x] = s1;
+  red[C::NT + threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = C::NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+      red[C::NT + threadIdx.x] += red[C::NT + threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = red[0];
+    partials[2 * blockIdx.x + 1] = red[C::NT];
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_sigma_finalize(const double *__restrict__ partials, int nparts,
+                                                         double ncells, double *__restrict__ out)
+{
+  __shared__ double r1[1024], r2[1024];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < nparts; i += 1024) {
+    a += partials[2 * i];
+    b += partials[2 * i + 1];
+  }
+  r1[threadIdx.x] = a;
+  r2[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double mean = r1[0] / ncells;
+    out[0] = r2[0] / ncells - mean * mean;  // sigma2_gauss (fourier.c:74)
+    out[1] = mean;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------
+template <typename T, int N>
+static hipError_t launch_fields_n(cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
+                                  double *partials, double *sig, hipStream_t s, hipEvent_t *ev)
+{
+  using CC = ColCfg<T, N>;
+  using RC = RowCfg<T, N>;
+  dim3 gcol(N / CC::TX, N);
+  if (ev) (void)hipEventRecord(ev[0], s);
+  hipLaunchKernelGGL((k_fill_z<T, N>), gcol, dim3(CC::NT), 0, s, F, f, tw);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  hipLaunchKernelGGL((k_fft_y<T, N>), gcol, dim3(CC::NT), 0, s, F, tw);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  int nrowblk = (N * N) / RC::TY;
+  hipLaunchKernelGGL((k_fft_x<T, N>), dim3(nrowblk), dim3(RC::NT), 0, s, F, tw, norm, partials);
+  if (ev) (void)hipEventRecord(ev[3], s);
+  hipLaunchKernelGGL(k_sigma_finalize, dim3(1), dim3(1024), 0, s, partials, nrowblk,
+                     (double)N * N * N, sig);
+  if (ev) (void)hipEventRecord(ev[4], s);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
+                         double *partials, double *sig, hipStream_t s, hipEvent_t *ev)
+{
+  switch (n) {
+#define CRIME_CASE(NN) \
+  case NN: return launch_fields_n<T, NN>(F, f, tw, norm, partials, sig, s, ev);
+    CRIME_FOR_EACH_N(CRIME_CASE)
+#undef CRIME_CASE
+  default: return hipErrorInvalidValue;
+  }
+}
+
+template hipError_t launch_fields<float>(int, cx<float> *, const FillParams &, const cx<float> *, double,
+                                         double *, double *, hipStream_t, hipEvent_t *);
+template hipError_t launch_fields<double>(int, cx<double> *, const FillParams &, const cx<double> *,
+                                          double, double *, double *, hipStream_t, hipEvent_t *);
+
+int fields_partials_count(int n, int prec)
+{
+  switch (n) {
+#define CRIME_CASE(NN) \
+  case NN: return prec ? (NN * NN) / RowCfg<double, NN>::TY : (NN * NN) / RowCfg<float, NN>::TY;
+    CRIME_FOR_EACH_N(CRIME_CASE)
+#undef CRIME_CASE
+  default: return 0;
+  }
+}
+
+}  // namespace crime

@@ -1,0 +1,190 @@
+/*
+ * crime_gh.h -- C ABI of the MI355X GetHI density-to-sky library (libcrime_gh.so).
+ *
+ * The reference GetHI is a monolithic CLI whose hot path is three in-process calls that mutate a
+ * ParamGetHI record in place (src/main_gh.c:52-62):
+ *     create_d_and_vr_fields(par);   src/fourier.c:375-438
+ *     get_HI(par);                   src/grid_tools.c:103-153
+ *     mk_T_maps(par);                src/pixelize.c:303-439 (scatter :343-384, mK scaling :399-414)
+ * This header is the device-side replacement of that seam.  Everything the three calls read from
+ * ParamGetHI (src/common_gh.h:138-209) is handed over once, by value, in crime_gh_desc; the library
+ * keeps the fields and maps resident in HBM and returns the maps in the layout of maps_HI after
+ * pixelize.c:412 (maps[ipix + N_pix*inu], RING ordering, channel 0 = lowest frequency, mK).
+ *
+ * Conventions (differences from the reference are deliberate and listed):
+ *  - every entry point returns a status code (0 = OK, <0 = error); nothing calls exit().  The
+ *    reference exits from report_error (src/common_gh.c:104-120) and my_malloc (src/common.c:31-51).
+ *  - one handle owns one HIP stream and its device buffers; handles are not shared across threads.
+ *  - the caller owns every host pointer; the descriptor's arrays are copied during crime_gh_create
+ *    and may be freed afterwards.
+ *  - tables are FP64 and are produced by the host library (gethi_host.h) exactly as cosmo_set
+ *    (src/cosmo.c:341-413) lays them out: NZ=5001 entries, uniform-r grid r_i = i/idr.
+ */
+#ifndef CRIME_GH_H
+#define CRIME_GH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRIME_GH_ABI_VERSION 1u
+#define CRIME_GH_NZ 5001            /* table length, NZ in src/common_gh.h:114 */
+#define CRIME_GH_DZ 0.001           /* z step of the z->r table, DZ in src/common_gh.h:113 */
+#define CRIME_GH_NU_21 1420.40575177 /* MHz, NU_21 in src/common.h:38 */
+
+/* status codes */
+enum {
+  CRIME_GH_OK = 0,
+  CRIME_GH_EINVAL = -1,   /* malformed descriptor or argument */
+  CRIME_GH_ENOMEM = -2,   /* device allocation failed */
+  CRIME_GH_EHIP = -3,     /* a HIP runtime call or kernel launch failed */
+  CRIME_GH_ENODEV = -4,   /* no usable gfx950 device */
+  CRIME_GH_ESTATE = -5,   /* call out of order (e.g. maps before fields) */
+  CRIME_GH_EABI = -6      /* abi_version mismatch */
+};
+
+/* arithmetic of the grids, the FFT and the maps.  FP32 mirrors the reference's _SPREC build
+ * (Makefile:10-20); FP64 mirrors the double build.  Per-cell geometry, tables, sigma^2 and
+ * ang2pix are FP64 in both. */
+enum { CRIME_GH_FP32 = 0, CRIME_GH_FP64 = 1 };
+
+/* where the unit-variance Fourier modes come from */
+enum {
+  CRIME_GH_MODES_PHILOX = 0,   /* generated on the device, counter-based (Philox-4x32-10) */
+  CRIME_GH_MODES_INJECTED = 1  /* unit_modes array supplied by the caller (parity runs) */
+};
+
+/* frequency binning: table of edges (reference build flag _IRREGULAR_NUTABLE, get_inu
+ * src/pixelize.c:181-208) or uniform bins with the reference's (int) truncation (:369) */
+enum { CRIME_GH_NU_TABLE = 0, CRIME_GH_NU_UNIFORM = 1 };
+
+/* HI bias model.  REFERENCE = bias_HI(z)=0.904+0.135(1+z)^1.696 (src/user_defined.c:32-35);
+ * CONST = b0; LINEAR = b0 + b1 (1+z).  The HI fraction is always fraction_HI (:27-30). */
+enum { CRIME_GH_BIAS_REFERENCE = 0, CRIME_GH_BIAS_CONST = 1, CRIME_GH_BIAS_LINEAR = 2 };
+
+typedef struct crime_gh_desc {
+  uint32_t abi_version;       /* CRIME_GH_ABI_VERSION */
+  int32_t precision;          /* CRIME_GH_FP32 or CRIME_GH_FP64 */
+  int32_t n_grid;             /* N: 2^a 3^b, 8 <= N <= 2048 */
+  int32_t n_nu;               /* number of channels */
+  int64_t n_side;             /* HEALPix Nside (power of two not required) */
+  int32_t nu_mode;            /* CRIME_GH_NU_TABLE or CRIME_GH_NU_UNIFORM */
+  int32_t rsd;                /* 1: add the RSD shift (reference), 0: real-space maps */
+  const double *nu0;          /* [n_nu] lower channel edges, ascending (TABLE mode) */
+  const double *nuf;          /* [n_nu] upper channel edges (TABLE mode) */
+  double nu_min, nu_max;      /* MHz */
+  double l_box;               /* box side, Mpc/h */
+  double pos_obs[3];          /* observer position, Mpc/h */
+  double omega_b, hhub;       /* for the mass-to-temperature factor (src/pixelize.c:308) */
+  double f0h0;                /* fgrowth_0 * hubble_0 (src/fourier.c:255) */
+  double r2_smooth;           /* squared smoothing scale (already squared, src/io_gh.c:255-260) */
+  int32_t do_smoothing;
+  int32_t nz;                 /* table length, must be CRIME_GH_NZ */
+  const double *z_z2r;        /* [nz] z_i = i*DZ                  (z_arr_z2r) */
+  const double *r_z2r;        /* [nz] chi(z_i)                     (r_arr_z2r) */
+  const double *z_r2z;        /* [nz] z(r_i)                       (z_arr_r2z) */
+  const double *r_r2z;        /* [nz] r_i = i/idr                  (r_arr_r2z) */
+  const double *d_r2z;        /* [nz] D(r_i)/D(0)                  (growth_d_arr) */
+  const double *v_r2z;        /* [nz] D H f/(f0 H0) at r_i         (growth_v_arr) */
+  double idr;                 /* glob_idr */
+  const double *logk;         /* [numk] log10 k of the P(k) table  (logkarr) */
+  const double *pk;           /* [numk] sigma8-normalised P(k)      (pkarr) */
+  int32_t numk;
+  double logkmin, logkmax, idlogk, n_s;
+  int32_t bias_model;         /* CRIME_GH_BIAS_* */
+  double bias_b0, bias_b1;
+  uint32_t seed;              /* Philox key for CRIME_GH_MODES_PHILOX */
+  int32_t n_sub;              /* samples per cell (reference N_SUBPART=10, src/pixelize.c:178) */
+  const double *sub_off;      /* [n_sub*3] sample offsets from the cell centre, Mpc/h */
+  int32_t mode_source;        /* CRIME_GH_MODES_* */
+  const double *unit_modes;   /* [2*N*N*(N/2+1)] interleaved (re,im), index kx+(N/2+1)(ky+N kz);
+                                 unit-variance modes of the reference's fill order
+                                 (src/fourier.c:260-302), used when mode_source is INJECTED */
+  int32_t device;             /* HIP device ordinal */
+} crime_gh_desc;
+
+/* per-call timings (hipEvent, ms) and diagnostics */
+typedef struct crime_gh_stats {
+  double ms_fill_fft_z;       /* mode fill fused with the kz pass */
+  double ms_fft_y;
+  double ms_fft_x;            /* kx pass + normalisation + FP64 sigma^2 partials */
+  double ms_sigma2;           /* final FP64 reduction */
+  double ms_maps_clear;
+  double ms_pixelize;         /* get_HI + RSD + sub-cell ang2pix + (nu,pix) scatter + mK scale */
+  double ms_total;
+  double sigma2_gauss;        /* <delta^2> - <delta>^2 (src/fourier.c:24-76) */
+  double mean_gauss;
+} crime_gh_stats;
+
+typedef struct crime_gh crime_gh_t;
+
+/* Grids exchanged with the parity accessors: N^3 FP64 values, x fastest (index x + N(y + N z)),
+ * i.e. the reference's padded arrays with the padding dropped. */
+enum {
+  CRIME_GH_GRID_DENS = 0,   /* normalised Gaussian delta   (grid_dens after fourier.c:409) */
+  CRIME_GH_GRID_VPOT = 1,   /* normalised velocity potential (grid_vpot after :410) */
+  CRIME_GH_GRID_RVEL = 2,   /* radial velocity (grid_rvel after fourier.c:368), read only */
+  CRIME_GH_GRID_MASS = 3,   /* HI mass per cell (grid_dens after grid_tools.c:145), read only */
+  CRIME_GH_GRID_DZRSD = 4   /* RSD redshift shift (grid_rvel after grid_tools.c:146), read only */
+};
+
+/* sizeof(crime_gh_desc) as compiled into the library, for FFI bindings to check their layout. */
+size_t crime_gh_desc_size(void);
+
+/* Allocates the device arena, uploads tables; no compute.  Replaces read_run_params' allocation
+ * part (init_fftw src/fourier.c:101-199, allocate_maps src/io_gh.c:60-67). */
+int crime_gh_create(const crime_gh_desc *desc, crime_gh_t **out);
+
+/* Frees everything; NULL is accepted.  Replaces param_gethi_free (src/io_gh.c:298-324). */
+void crime_gh_destroy(crime_gh_t *h);
+
+/* Message of the last failed call on h (h may be NULL for a failed crime_gh_create). */
+const char *crime_gh_last_error(const crime_gh_t *h);
+
+/* create_d_and_vr_fields (src/fourier.c:375-438): mode fill, 3-D inverse FFT of delta and of the
+ * velocity potential, normalisation by (sqrt(2 pi)/L)^3 and the FP64 sigma^2 of delta.  The radial
+ * velocity stencil (:307-373) is evaluated inside crime_gh_mk_T_maps. */
+int crime_gh_create_d_and_vr_fields(crime_gh_t *h, crime_gh_stats *st);
+
+/* get_HI (src/grid_tools.c:103-153) fused with mk_T_maps (src/pixelize.c:303-414): per cell the
+ * radial velocity, lognormal HI mass and RSD shift; per sample the channel and RING pixel; the
+ * scatter-add into zeroed maps, scaled to mK per channel. */
+int crime_gh_mk_T_maps(crime_gh_t *h, crime_gh_stats *st);
+
+/* Both stages in order (src/main_gh.c:52-62). */
+int crime_gh_run(crime_gh_t *h, crime_gh_stats *st);
+
+/* Device-to-host copy of the maps as float (what write_maps writes, src/io_gh.c:69-82) or double. */
+int crime_gh_copy_maps(crime_gh_t *h, float *maps_out);
+int crime_gh_copy_maps_f64(crime_gh_t *h, double *maps_out);
+
+/* sigma^2 and mean of the Gaussian field from the last create_d_and_vr_fields. */
+int crime_gh_get_sigma2(crime_gh_t *h, double *sigma2, double *mean);
+
+/* Parity accessors: read a grid (computing RVEL/MASS/DZRSD on demand from the resident fields),
+ * or overwrite DENS/VPOT (and sigma^2) so the map stage can be fed the oracle's fields. */
+int crime_gh_copy_grid(crime_gh_t *h, int which, double *dst);
+int crime_gh_set_grid(crime_gh_t *h, int which, const double *src);
+int crime_gh_set_sigma2(crime_gh_t *h, double sigma2, double mean);
+
+/* Per-sample channel and pixel of cells [cell0, cell0+ncell) in x-fastest order, n_sub samples
+ * each: inu[c*n_sub+s] (-1 when the sample falls outside every channel) and ipix[...] (-1 then). */
+int crime_gh_sample_indices(crime_gh_t *h, int64_t cell0, int64_t ncell, int32_t *inu,
+                            int64_t *ipix);
+
+/* Device pointer and size of the resident maps (n_nu*12*Nside^2 elements of the grid precision),
+ * for callers that hand the maps to further device work without a host round trip. */
+int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes);
+
+/* Batched run: every handle's fields and maps in one captured hipGraph launch on hs[0]'s device.
+ * Handles may differ in N, Nside, channels, cosmology and seed.  No counterpart in the reference
+ * (which runs one realisation per process); st may be NULL or an array of n. */
+int crime_gh_run_batch(crime_gh_t **hs, int n, crime_gh_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRIME_GH_H */

@@ -1,0 +1,176 @@
+"""ORACLE -- test infrastructure only: ctypes wrapper of oracle/_build/liboracle.so, the FP64 C
+restatement of the reference GetHI path.  Imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU baseline; never by crime_amd."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        from crime_amd._ffi import CrimeGhDesc
+        D = C.POINTER(CrimeGhDesc)
+        L.orc_fill_modes.argtypes = [D, _dp, _dp]
+        L.orc_ifft3d.argtypes = [C.c_int, _dp]
+        L.orc_ifft3d.restype = C.c_int
+        L.orc_c2r_3d.argtypes = [C.c_int, _dp, _dp]
+        L.orc_c2r_3d.restype = C.c_int
+        L.orc_sigma2.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_fields.argtypes = [D, _dp, _dp, _dp, _dp]
+        L.orc_fields.restype = C.c_int
+        L.orc_radial_velocity.argtypes = [D, _dp, _dp]
+        L.orc_get_hi.argtypes = [D, _dp, _dp, C.c_double, _dp, _dp]
+        L.orc_vec2pix_ring.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double]
+        L.orc_vec2pix_ring.restype = C.c_int64
+        L.orc_get_inu.argtypes = [D, C.c_double, C.c_int]
+        L.orc_get_inu.restype = C.c_int
+        L.orc_channel_factors.argtypes = [D, _dp]
+        L.orc_sample_indices.argtypes = [D, _dp, C.c_int64, C.c_int64, C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int64)]
+        L.orc_mk_T_maps.argtypes = [D, _dp, _dp, _dp]
+        L.orc_mk_T_maps.restype = C.c_int
+        L.orc_run.argtypes = [D, _dp, _dp, _dp]
+        L.orc_run.restype = C.c_int
+        L.orc_pk_linear0.argtypes = [D, C.c_double]
+        L.orc_pk_linear0.restype = C.c_double
+        for f in ("orc_z_of_r", "orc_dgrowth_of_r", "orc_vgrowth_of_r", "orc_r_of_z", "orc_bias"):
+            getattr(L, f).argtypes = [D, C.c_double]
+            getattr(L, f).restype = C.c_double
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def fill_modes(desc):
+    n = desc.n_grid
+    shape = (n, n, n // 2 + 1, 2)
+    dk = np.empty(shape)
+    pk = np.empty(shape)
+    lib().orc_fill_modes(C.byref(desc), _p(dk), _p(pk))
+    return dk, pk
+
+
+def c2r_3d(half):
+    """half: (N, N, N/2+1) complex [kz, ky, kx] -> (N, N, N) real [z, y, x]."""
+    n = half.shape[0]
+    h = np.ascontiguousarray(half, dtype=np.complex128)
+    out = np.empty((n, n, n))
+    rc = lib().orc_c2r_3d(n, h.ctypes.data_as(_dp), _p(out))
+    assert rc == 0
+    return out
+
+
+def ifft3d(full):
+    a = np.ascontiguousarray(full, dtype=np.complex128).copy()
+    rc = lib().orc_ifft3d(a.shape[0], a.ctypes.data_as(_dp))
+    assert rc == 0
+    return a
+
+
+def fields(desc):
+    n = desc.n_grid
+    dens = np.empty((n, n, n))
+    vpot = np.empty((n, n, n))
+    s2, m = C.c_double(), C.c_double()
+    rc = lib().orc_fields(C.byref(desc), _p(dens), _p(vpot), C.byref(s2), C.byref(m))
+    assert rc == 0
+    return dens, vpot, s2.value, m.value
+
+
+def sigma2(dens):
+    s2, m = C.c_double(), C.c_double()
+    d = np.ascontiguousarray(dens, dtype=np.float64)
+    lib().orc_sigma2(d.shape[0], _p(d), C.byref(s2), C.byref(m))
+    return s2.value, m.value
+
+
+def radial_velocity(desc, vpot):
+    out = np.empty_like(vpot, dtype=np.float64)
+    lib().orc_radial_velocity(C.byref(desc), _p(np.ascontiguousarray(vpot, np.float64)), _p(out))
+    return out
+
+
+def get_hi(desc, dens, rvel, sigma2):
+    mass = np.empty(dens.shape)
+    dz = np.empty(dens.shape)
+    lib().orc_get_hi(C.byref(desc), _p(np.ascontiguousarray(dens, np.float64)),
+                     _p(np.ascontiguousarray(rvel, np.float64)), sigma2, _p(mass), _p(dz))
+    return mass, dz
+
+
+def vec2pix_ring(nside, x, y, z):
+    return lib().orc_vec2pix_ring(nside, x, y, z)
+
+
+def get_inu(desc, nu, start=0):
+    return lib().orc_get_inu(C.byref(desc), nu, start)
+
+
+def channel_factors(desc):
+    out = np.empty(desc.n_nu)
+    lib().orc_channel_factors(C.byref(desc), _p(out))
+    return out
+
+
+def sample_indices(desc, dzrsd, cell0, ncell):
+    inu = np.empty((ncell, desc.n_sub), np.int32)
+    ipix = np.empty((ncell, desc.n_sub), np.int64)
+    lib().orc_sample_indices(C.byref(desc), _p(np.ascontiguousarray(dzrsd, np.float64)), cell0, ncell,
+                             inu.ctypes.data_as(C.POINTER(C.c_int32)), ipix.ctypes.data_as(C.POINTER(C.c_int64)))
+    return inu, ipix
+
+
+def mk_T_maps(desc, mass, dz):
+    maps = np.empty((desc.n_nu, 12 * desc.n_side ** 2))
+    rc = lib().orc_mk_T_maps(C.byref(desc), _p(np.ascontiguousarray(mass, np.float64)),
+                             _p(np.ascontiguousarray(dz, np.float64)), _p(maps))
+    assert rc == 0
+    return maps
+
+
+def run(desc):
+    maps = np.empty((desc.n_nu, 12 * desc.n_side ** 2))
+    s2, m = C.c_double(), C.c_double()
+    rc = lib().orc_run(C.byref(desc), _p(maps), C.byref(s2), C.byref(m))
+    assert rc == 0
+    return maps, s2.value, m.value
+
+
+def philox(ctr, k0, k1):
+    c = (C.c_uint32 * 4)(*ctr)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k0, k1, o)
+    return list(o)
+
+
+def pk_linear0(desc, lgk):
+    return lib().orc_pk_linear0(C.byref(desc), lgk)
+
+
+def z_of_r(desc, r):
+    return lib().orc_z_of_r(C.byref(desc), r)
+
+
+def num_threads():
+    return lib().orc_num_threads()
