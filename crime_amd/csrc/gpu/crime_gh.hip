@@ -83,6 +83,36 @@ static double r_of_z_host(const crime_gh_desc *d, double z)
   return d->r_z2r[iz] + (d->r_z2r[iz + 1] - d->r_z2r[iz]) * (z - d->z_z2r[iz]) / CRIME_GH_DZ;
 }
 
+static double pk_host(const crime_gh_desc *d, double lgk)
+{
+  int ik = (int)((lgk - d->logkmin) * d->idlogk);
+  if (ik < 0) return d->pk[0] * pow(10, d->n_s * (lgk - d->logkmin));
+  if (ik < d->numk - 1) return d->pk[ik] + (lgk - d->logk[ik]) * (d->pk[ik + 1] - d->pk[ik]) * d->idlogk;
+  return d->pk[d->numk - 1] * pow(10, -3 * (lgk - d->logkmax));
+}
+
+// rms(delta)/rms(phi) of the grid's modes, estimated with a spherical k-space integral from the
+// fundamental to the radius of the sphere with the grid cube's volume.  Only its order of magnitude
+// matters: phi is packed as phi_scale*phi next to delta in one complex FP32 element, so that both
+// fields carry rounding errors relative to their own size.
+static double phi_balance(const crime_gh_desc *d)
+{
+  double dk = 2 * M_PI / d->l_box;
+  double kmax = 0.5 * d->n_grid * dk * cbrt(6 / M_PI);
+  const int nk = 4000;
+  double l0 = log(dk), l1 = log(kmax), i0 = 0, i2 = 0;
+  for (int i = 0; i < nk; i++) {
+    double k = exp(l0 + (l1 - l0) * (i + 0.5) / nk);
+    double p = pk_host(d, log10(k)) * k * k * k;  // k^2 P(k) dk with dk = k dlnk
+    if (d->do_smoothing) p *= exp(-d->r2_smooth * k * k);
+    double psi = d->f0h0 / (k * k);
+    i0 += p;
+    i2 += p * psi * psi;
+  }
+  if (!(i0 > 0) || !(i2 > 0)) return 1.0;
+  return sqrt(i0 / i2);
+}
+
 static int validate(const crime_gh_desc *d)
 {
   if (!d) return set_error(nullptr, CRIME_GH_EINVAL, "descriptor is NULL");
@@ -270,6 +300,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   f.mode_source = d->mode_source;
   f.unit_modes = h->d_modes;
   h->norm = pow(sqrt(2 * M_PI) / d->l_box, 3);  // fourier.c:403
+  f.phi_scale = phi_balance(d);
 
   // map parameters
   MapParams &m = h->map;

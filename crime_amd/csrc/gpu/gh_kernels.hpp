@@ -64,6 +64,7 @@ struct RowCfg {
 struct FillParams {
   int n;
   double dk, idk3, f0h0, r2_smooth;
+  double phi_scale;  // phi is packed as phi_scale*phi so both halves of F have similar magnitude
   int do_smoothing;
   const double *logk, *pk;
   int numk;

@@ -71,7 +71,7 @@ __device__ __forceinline__ cx<T> fill_element(const FillParams &f, int kx, int k
   double sigma2 = pk_linear0(f, 0.5 * log10(k2)) * f.idk3;
   if (f.do_smoothing) sigma2 *= exp(-f.r2_smooth * k2);
   double amp = sqrt(sigma2);
-  double psi = f.f0h0 / k2;
+  double psi = f.phi_scale * (f.f0h0 / k2);
   int myn = (n - ky) % n, mzn = (n - kz) % n;  // -k in y and z
   cx<double> g;
   if (kx > 0 && kx < h) {
@@ -129,7 +129,8 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_fft_y(cx<T> *__restrict_
 template <typename T, int N>
 __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_fft_x(cx<T> *__restrict__ F,
                                                             const cx<T> *__restrict__ tw,
-                                                            double norm, double *__restrict__ partials)
+                                                            double norm, double norm_phi,
+                                                            double *__restrict__ partials)
 {
   using C = RowCfg<T, N>;
   __shared__ cx<T> lds[N * C::TY];
@@ -140,7 +141,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_fft_x(cx<T> *__restrict_
   double s1 = 0, s2 = 0;
   auto load = [&](int e) { return line[e]; };
   auto store = [&](int e, cx<T> v) {
-    cx<T> o = {(T)((double)v.x * norm), (T)((double)v.y * norm)};
+    cx<T> o = {(T)((double)v.x * norm), (T)((double)v.y * norm_phi)};
     double d = (double)o.x;
     s1 += d;
     s2 += d * d;
@@ -206,7 +207,8 @@ static hipError_t launch_fields_n(cx<T> *F, const FillParams &f, const cx<T> *tw
   hipLaunchKernelGGL((k_fft_y<T, N>), gcol, dim3(CC::NT), 0, s, F, tw);
   if (ev) (void)hipEventRecord(ev[2], s);
   int nrowblk = (N * N) / RC::TY;
-  hipLaunchKernelGGL((k_fft_x<T, N>), dim3(nrowblk), dim3(RC::NT), 0, s, F, tw, norm, partials);
+  hipLaunchKernelGGL((k_fft_x<T, N>), dim3(nrowblk), dim3(RC::NT), 0, s, F, tw, norm, norm / f.phi_scale,
+                     partials);
   if (ev) (void)hipEventRecord(ev[3], s);
   hipLaunchKernelGGL(k_sigma_finalize, dim3(1), dim3(1024), 0, s, partials, nrowblk,
                      (double)N * N * N, sig);
