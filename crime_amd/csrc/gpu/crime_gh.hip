@@ -26,6 +26,7 @@ struct crime_gh {
   double *d_pk = nullptr;      // logk | pk
   double *d_edges = nullptr;   // [n_nu+1]
   double *d_prefac = nullptr;  // [n_nu]
+  int *d_guess = nullptr;      // channel guess table
   double *d_sub = nullptr;     // [3*n_sub]
   double *d_modes = nullptr;   // injected unit modes
   void *d_tw = nullptr;        // twiddles exp(+2 pi i m/N)
@@ -169,7 +170,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_sub,
+  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_guess, h->d_sub,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -269,6 +270,22 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   }
   HIP_OR_FAIL(h, hipMemcpy(h->d_edges, edges.data(), edges.size() * sizeof(double), hipMemcpyHostToDevice));
   HIP_OR_FAIL(h, hipMemcpy(h->d_prefac, prefac.data(), prefac.size() * sizeof(double), hipMemcpyHostToDevice));
+  // channel guess table: guess[j] = channel holding g_lo + j/g_inv (table mode)
+  int n_guess = 8 * d->n_nu < 64 ? 64 : 8 * d->n_nu;
+  double g_lo = edges[0], g_inv = n_guess / (edges[d->n_nu] - edges[0]);
+  std::vector<int> guess(n_guess);
+  for (int j = 0; j < n_guess; j++) {
+    double nu = g_lo + j / g_inv;
+    int lo = 0, hi = d->n_nu;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) / 2;
+      if (nu >= edges[mid]) lo = mid;
+      else hi = mid;
+    }
+    guess[j] = lo;
+  }
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_guess, n_guess * sizeof(int)));
+  HIP_OR_FAIL(h, hipMemcpy(h->d_guess, guess.data(), n_guess * sizeof(int), hipMemcpyHostToDevice));
 
   if (h->prec) {
     std::vector<cx<double>> tw;
@@ -344,6 +361,10 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     m.nu_hi = d->nu_max;
   }
   m.prefac = h->d_prefac;
+  m.guess = h->d_guess;
+  m.n_guess = n_guess;
+  m.g_lo = g_lo;
+  m.g_inv = g_inv;
   m.nside = h->nside;
   m.npix = h->npix;
   m.sig = h->d_sig;

@@ -130,6 +130,23 @@ __device__ __forceinline__ int channel_of(double nu, int n_nu, int nu_mode, cons
   return lo;
 }
 
+// Same result as channel_of, starting from a guess table over [g_lo, g_lo + n_guess/g_inv):
+// guess[j] is the channel of g_lo + j/g_inv, so the walk below takes at most a step or two.
+__device__ __forceinline__ int channel_of_guess(double nu, int n_nu, int nu_mode, const double *edges,
+                                                double nu_min, double inv_dnu, const int *guess,
+                                                int n_guess, double g_lo, double g_inv)
+{
+#pragma clang fp contract(off)
+  if (nu_mode == 1) return channel_of(nu, n_nu, nu_mode, edges, nu_min, inv_dnu);
+  if (!(nu >= edges[0]) || !(nu < edges[n_nu])) return -1;
+  double t = (nu - g_lo) * g_inv;
+  int j = t < 0 ? 0 : (t >= (double)(n_guess - 1) ? n_guess - 1 : (int)t);
+  int inu = guess[j];
+  while (inu > 0 && nu < edges[inu]) inu--;
+  while (inu < n_nu - 1 && nu >= edges[inu + 1]) inu++;
+  return inu;
+}
+
 // ---------------------------------------------------------------------------------------------
 // RING pixel of a (non-normalised) direction, Gorski et al. 2005 (ApJ 622:759) section 4; vlen is
 // the vector's length sqrt(x*x+y*y+z*z).  Near the poles (|cos theta| > 0.99) sin theta is taken

@@ -92,7 +92,14 @@ struct MapParams {
   const double *prefac;    // [n_nu] mK factor per channel (pixelize.c:399-409)
   int64_t nside, npix;
   const double *sig;       // device [sigma2, mean]
+  const int *guess;        // [n_guess] channel of g_lo + j/g_inv (table mode)
+  int n_guess;
+  double g_lo, g_inv;
 };
+
+// LDS aggregation of the scatter: one workgroup per TILE_X x TILE_Y x TILE_Z cells, AGG_SLOTS bins
+constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 4;
+constexpr int AGG_SLOTS = 4096;
 
 template <typename T>
 hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,

@@ -88,7 +88,8 @@ __device__ __forceinline__ int sample_bin(const MapParams &m, const CellState &s
   double r = sqrt(x * x + y * y + z * z);
   double redshift = z_of_r(m.tab, r) + s.dz;
   double nu = 1420.40575177 / (1 + redshift);
-  int inu = channel_of(nu, m.n_nu, m.nu_mode, m.edges, m.nu_min, m.inv_dnu);
+  int inu = channel_of_guess(nu, m.n_nu, m.nu_mode, m.edges, m.nu_min, m.inv_dnu, m.guess, m.n_guess,
+                             m.g_lo, m.g_inv);
   if (!(r > 0)) inu = -1;  // a sample on the observer has no direction
   ipix = inu >= 0 ? vec2pix_ring(m.nside, x, y, z, r) : -1;
   return inu;
@@ -107,26 +108,76 @@ __device__ __forceinline__ bool cell_may_bin(const MapParams &m, const CellState
   return true;
 }
 
+// Add val to bin `key` of the workgroup's LDS table (open addressing, linear probing); a sample that
+// finds no slot within the probe budget goes straight to the map in HBM.
+template <typename T>
+__device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned long long *keys, T *vals,
+                                        T *__restrict__ maps)
+{
+  const unsigned long long EMPTY = ~0ull;
+  unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 52) & (AGG_SLOTS - 1);
+  for (int p = 0; p < 64; p++) {
+    unsigned long long k = keys[h];
+    if (k == EMPTY) {
+      unsigned long long old = atomicCAS(&keys[h], EMPTY, key);
+      if (old == EMPTY || old == key) {
+        atomicAdd(&vals[h], val);
+        return;
+      }
+    }
+    else if (k == key) {
+      atomicAdd(&vals[h], val);
+      return;
+    }
+    h = (h + 1) & (AGG_SLOTS - 1);
+  }
+  unsafeAtomicAdd(maps + key, val);
+}
+
+// One workgroup = one TILE_X x TILE_Y x TILE_Z block of cells (one cell per thread).  Samples are
+// summed per (channel, pixel) bin in LDS and every occupied bin is added to HBM once, so the
+// global atomics scale with the bins a tile touches instead of with its samples.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T *__restrict__ maps,
-                                                  MapParams m, size_t ncell)
+                                                  MapParams m, int ntx, int nty)
 {
-  size_t cell = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (cell >= ncell) return;
-  int ix, iy, iz;
-  cell_coords<T>(m, cell, ix, iy, iz);
+  __shared__ unsigned long long keys[AGG_SLOTS];
+  __shared__ T vals[AGG_SLOTS];
+  const int n = m.n;
+  const int t = threadIdx.x;
+  size_t b = blockIdx.x;
+  const int tx = (int)(b % ntx), ty = (int)((b / ntx) % nty), tz = (int)(b / ((size_t)ntx * nty));
+  const int ix = tx * TILE_X + (t % TILE_X), iy = ty * TILE_Y + ((t / TILE_X) % TILE_Y),
+            iz = tz * TILE_Z + t / (TILE_X * TILE_Y);
   CellState s;
-  cell_velocity<T>(m, F, ix, iy, iz, s);
-  s.dz = m.rsd ? s.vr * vgrowth_of_r(m.tab, s.r) : 0.0;
-  if (!cell_may_bin(m, s)) return;
-  double delta = (double)F[cell].x;
-  cell_hi(m, delta, m.sig[0], s);
-  double mass_sub = s.mass / m.n_sub;
-  for (int isub = 0; isub < m.n_sub; isub++) {
-    int64_t ipix;
-    int inu = sample_bin(m, s, isub, ipix);
-    if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix)
-      unsafeAtomicAdd(maps + (size_t)ipix + (size_t)m.npix * inu, (T)(mass_sub * m.prefac[inu]));
+  bool active = ix < n && iy < n && iz < n;
+  if (active) {
+    cell_velocity<T>(m, F, ix, iy, iz, s);
+    s.dz = m.rsd ? s.vr * vgrowth_of_r(m.tab, s.r) : 0.0;
+    active = cell_may_bin(m, s);
+  }
+  if (!__syncthreads_or(active)) return;  // no cell of the tile reaches a channel
+  for (int i = t; i < AGG_SLOTS; i += 256) {
+    keys[i] = ~0ull;
+    vals[i] = (T)0;
+  }
+  __syncthreads();
+  if (active) {
+    size_t cell = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
+    cell_hi(m, (double)F[cell].x, m.sig[0], s);
+    double mass_sub = s.mass / m.n_sub;
+    for (int isub = 0; isub < m.n_sub; isub++) {
+      int64_t ipix;
+      int inu = sample_bin(m, s, isub, ipix);
+      if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix)
+        agg_add<T>((unsigned long long)ipix + (unsigned long long)m.npix * inu, (T)(mass_sub * m.prefac[inu]),
+                   keys, vals, maps);
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < AGG_SLOTS; i += 256) {
+    unsigned long long k = keys[i];
+    if (k != ~0ull) unsafeAtomicAdd(maps + k, vals[i]);
   }
 }
 
@@ -204,7 +255,9 @@ hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m
   if (zb > 65536u * 4) zb = 65536u * 4;
   hipLaunchKernelGGL(k_zero<T>, dim3(zb), dim3(256), 0, s, maps, nmaps);
   if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_pixelize<T>, dim3(nblk(ncell)), dim3(256), 0, s, F, maps, m, ncell);
+  int ntx = (m.n + TILE_X - 1) / TILE_X, nty = (m.n + TILE_Y - 1) / TILE_Y, ntz = (m.n + TILE_Z - 1) / TILE_Z;
+  size_t ntiles = (size_t)ntx * nty * ntz;
+  hipLaunchKernelGGL(k_pixelize<T>, dim3((unsigned)ntiles), dim3(256), 0, s, F, maps, m, ntx, nty);
   if (ev) (void)hipEventRecord(ev[2], s);
   return hipGetLastError();
 }
