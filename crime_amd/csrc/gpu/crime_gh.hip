@@ -27,6 +27,7 @@ struct crime_gh {
   double *d_edges = nullptr;   // [n_nu+1]
   double *d_prefac = nullptr;  // [n_nu]
   int *d_guess = nullptr;      // channel guess table
+  void *d_amp = nullptr;       // (A, A psi) per |k|^2
   double *d_sub = nullptr;     // [3*n_sub]
   double *d_modes = nullptr;   // injected unit modes
   void *d_tw = nullptr;        // twiddles exp(+2 pi i m/N)
@@ -170,7 +171,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_guess, h->d_sub,
+  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_guess, h->d_amp, h->d_sub,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -318,6 +319,11 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   f.unit_modes = h->d_modes;
   h->norm = pow(sqrt(2 * M_PI) / d->l_box, 3);  // fourier.c:403
   f.phi_scale = phi_balance(d);
+  HIP_OR_FAIL(h, hipMalloc(&h->d_amp, (size_t)amp_count(h->n) * 2 * esz));
+  f.amp = h->d_amp;
+  HIP_OR_FAIL(h, h->prec ? launch_amp_table<double>(f, h->d_amp, h->stream)
+                         : launch_amp_table<float>(f, h->d_amp, h->stream));
+  HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
 
   // map parameters
   MapParams &m = h->map;

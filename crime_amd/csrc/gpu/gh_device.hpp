@@ -152,18 +152,23 @@ __device__ __forceinline__ int channel_of_guess(double nu, int n_nu, int nu_mode
 // the vector's length sqrt(x*x+y*y+z*z).  Near the poles (|cos theta| > 0.99) sin theta is taken
 // from sqrt(x*x+y*y)/vlen to avoid the cancellation in 1-|cos theta|.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ int64_t imod64(int64_t v, int64_t m)
+// v mod m for v within a few multiples of m (no integer division)
+__device__ __forceinline__ int imod_small(int v, int m)
 {
-  int64_t r = v % m;
-  return r < 0 ? r + m : r;
+  while (v >= m) v -= m;
+  while (v < 0) v += m;
+  return v;
 }
 
-__device__ __forceinline__ int64_t vec2pix_ring(int64_t nside, double x, double y, double z, double vlen)
+// Integer work is 32-bit: for Nside <= 8192 every index below is < 12 Nside^2 < 2^31, and the
+// truncations are of non-negative values < 5 Nside, where (int) and (int64_t) casts agree.
+__device__ __forceinline__ int64_t vec2pix_ring(int64_t nside64, double x, double y, double z, double vlen)
 {
 #pragma clang fp contract(off)
   const double twopi = 6.283185307179586476925286766559;
   const double inv_halfpi = 0.636619772367581343075535053490057;
   const double twothird = 2.0 / 3.0;
+  const int nside = (int)nside64;
   double cth = z / vlen;
   double za = fabs(cth);
   double phi = atan2(y, x);
@@ -177,13 +182,13 @@ __device__ __forceinline__ int64_t vec2pix_ring(int64_t nside, double x, double 
   if (za <= twothird) {
     double temp1 = dn * (0.5 + tt);
     double temp2 = dn * cth * 0.75;
-    int64_t jp = (int64_t)(temp1 - temp2);
-    int64_t jm = (int64_t)(temp1 + temp2);
-    int64_t ir = nside + 1 + jp - jm;
-    int64_t kshift = 1 - (ir & 1);
-    int64_t ip = (jp + jm - nside + kshift + 1) / 2;
-    ip = imod64(ip, 4 * nside);
-    return nside * (nside - 1) * 2 + (ir - 1) * 4 * nside + ip;
+    int jp = (int)(temp1 - temp2);  // in [0, 5 Nside]
+    int jm = (int)(temp1 + temp2);
+    int ir = nside + 1 + jp - jm;
+    int kshift = 1 - (ir & 1);
+    int ip = (jp + jm - nside + kshift + 1) / 2;  // in [-Nside/2, 5 Nside + 1]
+    ip = imod_small(ip, 4 * nside);
+    return (int64_t)(nside * (nside - 1) * 2 + (ir - 1) * 4 * nside + ip);
   }
   double tp = tt - (double)(int)tt;
   double tmp;
@@ -194,12 +199,12 @@ __device__ __forceinline__ int64_t vec2pix_ring(int64_t nside, double x, double 
   else {
     tmp = dn * sqrt(3.0 * (1.0 - za));
   }
-  int64_t jp = (int64_t)(tp * tmp);
-  int64_t jm = (int64_t)((1.0 - tp) * tmp);
-  int64_t ir = jp + jm + 1;
-  int64_t ip = (int64_t)(tt * (double)ir);
-  ip = imod64(ip, 4 * ir);
-  if (cth > 0) return 2 * ir * (ir - 1) + ip;
-  return 12 * nside * nside - 2 * ir * (ir + 1) + ip;
+  int jp = (int)(tp * tmp);
+  int jm = (int)((1.0 - tp) * tmp);
+  int ir = jp + jm + 1;
+  int ip = (int)(tt * (double)ir);  // in [0, 4 ir]
+  ip = imod_small(ip, 4 * ir);
+  if (cth > 0) return (int64_t)(2 * ir * (ir - 1) + ip);
+  return (int64_t)(12 * nside * nside - 2 * ir * (ir + 1) + ip);
 }
 }  // namespace crime

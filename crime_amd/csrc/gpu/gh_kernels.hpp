@@ -33,7 +33,7 @@ constexpr int emax() { return sizeof(T) == 4 ? 64 : 32; }
 // strided (kz, ky) passes: TX adjacent columns per workgroup, TC threads per column
 template <typename T, int N>
 struct ColCfg {
-  static constexpr int TC = plan_tc(N, cmax(16, N / emax<T>()));
+  static constexpr int TC = plan_tc(N, cmax(64, N / emax<T>()));
   static constexpr int CB = N * (int)sizeof(cx<T>);
   static constexpr int TXMAX = cmin(16, cpow2_div(N));
   static constexpr int tx_for(int budget)
@@ -72,7 +72,14 @@ struct FillParams {
   uint32_t seed;
   int mode_source;
   const double *unit_modes;
+  const void *amp;  // cx<T>[amp_count(n)]: (A, A psi) for k^2 = m dk^2, filled by launch_amp_table
 };
+
+// entries of the amplitude table: every m = kx^2 + ky^2 + kz^2 with |k_i| <= N/2
+inline int amp_count(int n) { return 3 * (n / 2) * (n / 2) + 1; }
+
+template <typename T>
+hipError_t launch_amp_table(const FillParams &f, void *tab, hipStream_t s);
 
 struct MapParams {
   int n;
@@ -99,7 +106,7 @@ struct MapParams {
 
 // LDS aggregation of the scatter: one workgroup per TILE_X x TILE_Y x TILE_Z cells, AGG_SLOTS bins
 constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 4;
-constexpr int AGG_SLOTS = 4096;
+constexpr int AGG_SLOTS = 2048;
 
 template <typename T>
 hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,

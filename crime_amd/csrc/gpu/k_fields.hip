@@ -32,29 +32,67 @@ __device__ __forceinline__ double pk_linear0(const FillParams &f, double lgk)
   return f.pk[f.numk - 1] * pow(10.0, -3 * (lgk - f.logkmax));
 }
 
-// unit-variance complex mode at half-spectrum position (kx in [0,N/2], ky, kz)
-__device__ __forceinline__ cx<double> unit_mode(const FillParams &f, int kx, int ky, int kz)
+// Philox words of half-spectrum mode idx -> the two uniforms (53-bit) of rng_delta_gauss
+__device__ __forceinline__ void mode_uniforms(const FillParams &f, size_t idx, double &u1, double &u2)
 {
-  size_t idx = (size_t)kx + (size_t)(f.n / 2 + 1) * ((size_t)ky + (size_t)f.n * (size_t)kz);
-  if (f.mode_source == 1) return {f.unit_modes[2 * idx], f.unit_modes[2 * idx + 1]};
   u32x4 c;
   c.v[0] = (uint32_t)idx;
   c.v[1] = (uint32_t)(idx >> 32);
   c.v[2] = 0;
   c.v[3] = 0;
   u32x4 r = philox4x32_10(c, f.seed, PHILOX_KEY1);
-  double u1 = u53(r.v[0], r.v[1]);
-  double u2 = u53(r.v[2], r.v[3]);
-  // rng_delta_gauss (src/common.c:154-164): phase 2 pi u1, modulus sqrt(-ln(1-u2))
-  double mod = sqrt(-log(1.0 - u2));
-  double s, co;
-  sincos(2.0 * M_PI * u1, &s, &co);
-  return {mod * co, mod * s};
+  u1 = u53(r.v[0], r.v[1]);
+  u2 = u53(r.v[2], r.v[3]);
 }
 
-// F = delta + i phi at full-grid position (kx, ky, kz), kx in [0, N)
+// Unit-variance complex mode at half-spectrum position (kx in [0,N/2], ky, kz):
+// rng_delta_gauss (src/common.c:154-164), phase 2 pi u1 and modulus sqrt(-ln(1-u2)).  FP64 grids
+// evaluate it in FP64; FP32 grids evaluate the logarithm, root and sin/cos in FP32 from the exact
+// FP64 value of 1-u2 (relative error ~1e-7 per mode, the precision the mode is stored in).
 template <typename T>
-__device__ __forceinline__ cx<T> fill_element(const FillParams &f, int kx, int ky, int kz)
+__device__ __forceinline__ cx<T> unit_mode(const FillParams &f, int kx, int ky, int kz)
+{
+  size_t idx = (size_t)kx + (size_t)(f.n / 2 + 1) * ((size_t)ky + (size_t)f.n * (size_t)kz);
+  if (f.mode_source == 1) return {(T)f.unit_modes[2 * idx], (T)f.unit_modes[2 * idx + 1]};
+  double u1, u2;
+  mode_uniforms(f, idx, u1, u2);
+  if constexpr (sizeof(T) == 8) {
+    double mod = sqrt(-log(1.0 - u2));
+    double s, co;
+    sincos(2.0 * M_PI * u1, &s, &co);
+    return {mod * co, mod * s};
+  }
+  else {
+    float mod = sqrtf(-logf((float)(1.0 - u2)));
+    float s, co;
+    sincospif(2.0f * (float)u1, &s, &co);
+    return {mod * co, mod * s};
+  }
+}
+
+// (A, A psi) for k^2 = m dk^2: sqrt(P(k)/dk^3 [e^{-r_s^2 k^2}]) (fourier.c:292-295) and the
+// velocity-potential factor psi = phi_scale f0 H0 / k^2 (fourier.c:298).  Built once per handle.
+template <typename T>
+__global__ __launch_bounds__(256) void k_amp_table(FillParams f, cx<T> *__restrict__ tab, int mmax)
+{
+  int m = blockIdx.x * 256 + threadIdx.x;
+  if (m > mmax) return;
+  if (m == 0) {
+    tab[0] = {(T)0, (T)0};
+    return;
+  }
+  double k2 = (double)m * (f.dk * f.dk);
+  double sigma2 = pk_linear0(f, 0.5 * log10(k2)) * f.idk3;
+  if (f.do_smoothing) sigma2 *= exp(-f.r2_smooth * k2);
+  double amp = sqrt(sigma2);
+  double psi = f.phi_scale * (f.f0h0 / k2);
+  tab[m] = {(T)amp, (T)(amp * psi)};
+}
+
+// F = delta + i phi_scale phi at full-grid position (kx, ky, kz), kx in [0, N)
+template <typename T>
+__device__ __forceinline__ cx<T> fill_element(const FillParams &f, const cx<T> *__restrict__ amp, int kx,
+                                              int ky, int kz)
 {
   const int n = f.n, h = n / 2;
   if ((kx | ky | kz) == 0) return {(T)0, (T)0};
@@ -62,31 +100,22 @@ __device__ __forceinline__ cx<T> fill_element(const FillParams &f, int kx, int k
   int ax = kx <= h ? kx : n - kx;
   int ay = ky <= h ? ky : n - ky;
   int az = kz <= h ? kz : n - kz;
-  double kxd = ax * f.dk, kyd = ay * f.dk, kzd = az * f.dk;
-  double k2;
-  {
-#pragma clang fp contract(off)
-    k2 = kxd * kxd + kyd * kyd + kzd * kzd;
-  }
-  double sigma2 = pk_linear0(f, 0.5 * log10(k2)) * f.idk3;
-  if (f.do_smoothing) sigma2 *= exp(-f.r2_smooth * k2);
-  double amp = sqrt(sigma2);
-  double psi = f.phi_scale * (f.f0h0 / k2);
+  cx<T> ab = amp[ax * ax + ay * ay + az * az];
   int myn = (n - ky) % n, mzn = (n - kz) % n;  // -k in y and z
-  cx<double> g;
+  cx<T> g;
   if (kx > 0 && kx < h) {
-    g = unit_mode(f, kx, ky, kz);
+    g = unit_mode<T>(f, kx, ky, kz);
   }
   else if (kx > h) {
-    cx<double> m = unit_mode(f, n - kx, myn, mzn);
+    cx<T> m = unit_mode<T>(f, n - kx, myn, mzn);
     g = {m.x, -m.y};
   }
   else {  // kx == 0 or kx == N/2: Hermitian part of the plane
-    cx<double> a = unit_mode(f, kx, ky, kz), b = unit_mode(f, kx, myn, mzn);
-    g = {0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+    cx<T> a = unit_mode<T>(f, kx, ky, kz), b = unit_mode<T>(f, kx, myn, mzn);
+    g = {(T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)};
   }
-  double dx_ = amp * g.x, dy_ = amp * g.y;
-  return {(T)(dx_ - psi * dy_), (T)(dy_ + psi * dx_)};
+  // delta = A g, phi_scale phi = A psi g; F = delta + i phi_scale phi
+  return {ab.x * g.x - ab.y * g.y, ab.x * g.y + ab.y * g.x};
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -103,7 +132,8 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_fill_z(cx<T> *__restrict
   const size_t n2 = (size_t)N * N;
   cx<T> *col = F + (size_t)kx + (size_t)N * ky;
   // the generator is too large to unroll into the register tile: stage the modes in LDS first
-  for (int e = t; e < N; e += C::TC) lds[e * C::TX + c] = fill_element<T>(f, kx, ky, e);
+  const cx<T> *amp = (const cx<T> *)f.amp;
+  for (int e = t; e < N; e += C::TC) lds[e * C::TX + c] = fill_element<T>(f, amp, kx, ky, e);
   __syncthreads();
   auto load = [&](int e) { return lds[e * C::TX + c]; };
   auto store = [&](int e, cx<T> v) { col[(size_t)e * n2] = v; };
@@ -233,6 +263,16 @@ template hipError_t launch_fields<float>(int, cx<float> *, const FillParams &, c
                                          double *, double *, hipStream_t, hipEvent_t *);
 template hipError_t launch_fields<double>(int, cx<double> *, const FillParams &, const cx<double> *,
                                           double, double *, double *, hipStream_t, hipEvent_t *);
+
+template <typename T>
+hipError_t launch_amp_table(const FillParams &f, void *tab, hipStream_t s)
+{
+  int mmax = amp_count(f.n) - 1;
+  hipLaunchKernelGGL(k_amp_table<T>, dim3((mmax + 256) / 256), dim3(256), 0, s, f, (cx<T> *)tab, mmax);
+  return hipGetLastError();
+}
+template hipError_t launch_amp_table<float>(const FillParams &, void *, hipStream_t);
+template hipError_t launch_amp_table<double>(const FillParams &, void *, hipStream_t);
 
 int fields_partials_count(int n, int prec)
 {

@@ -115,7 +115,8 @@ __device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned 
                                         T *__restrict__ maps)
 {
   const unsigned long long EMPTY = ~0ull;
-  unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 52) & (AGG_SLOTS - 1);
+  constexpr int SHIFT = 64 - __builtin_ctz(AGG_SLOTS);  // Fibonacci hashing onto the table
+  unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> SHIFT) & (AGG_SLOTS - 1);
   for (int p = 0; p < 64; p++) {
     unsigned long long k = keys[h];
     if (k == EMPTY) {
