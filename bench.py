@@ -58,7 +58,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="production_1024", choices=WORKLOADS)
-    ap.add_argument("--cpu-n", type=int, default=256, help="grid of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-n", type=int, default=512, help="grid of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (scale reference for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the GetHI path on one GPU; multi-GPU slabs are a design note "
@@ -67,6 +69,16 @@ def main():
 
     import crime_amd
     import crime_amd.configs as cf
+
+    if args.calib_copy:
+        import torch
+
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("bench: calibration copy of 1 GiB done", file=sys.stderr, flush=True)
+        del src, dst
 
     with tempfile.TemporaryDirectory() as tmp:
         ini = getattr(cf, args.workload)(tmp)

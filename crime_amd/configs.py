@@ -49,8 +49,9 @@ def small_fp64_256(outdir, **kw):
     seed 1001 (the reference's CPU-runnable size)."""
     a = dict(n_grid=256, n_side=128, n_nu=64, nu_lo=700.0, nu_hi=800.0, l_box=4900.0, seed=1001,
              precision="double", bias_model="const", bias_b0=1.0, n_subcell=2)
+    name = kw.pop("name", "small_fp64_256")
     a.update(kw)
-    return make(outdir, kw.pop("name", "small_fp64_256"), **a)
+    return make(outdir, name, **a)
 
 
 def production_1024(outdir, **kw):
@@ -58,8 +59,9 @@ def production_1024(outdir, **kw):
     Nside 512, the reference's 10 shared random sub-cell samples, seed 2002."""
     a = dict(n_grid=1024, n_side=512, n_nu=200, nu_lo=400.0, nu_hi=800.0, l_box=8300.0, seed=2002,
              precision="single", bias_model="linear", bias_b0=0.7, bias_b1=0.3, n_subcell=0)
+    name = kw.pop("name", "production_1024")
     a.update(kw)
-    return make(outdir, kw.pop("name", "production_1024"), **a)
+    return make(outdir, name, **a)
 
 
 def largest_2048(outdir, **kw):
@@ -67,8 +69,9 @@ def largest_2048(outdir, **kw):
     sigma_8=0.83, n_s=0.965, seed 3003."""
     a = dict(n_grid=2048, n_side=1024, n_nu=400, nu_lo=400.0, nu_hi=800.0, l_box=8300.0, seed=3003,
              sigma_8=0.83, n_s=0.965, precision="single", n_subcell=4)
+    name = kw.pop("name", "largest_2048")
     a.update(kw)
-    return make(outdir, kw.pop("name", "largest_2048"), **a)
+    return make(outdir, name, **a)
 
 
 def ensemble_member(outdir, i, **kw):

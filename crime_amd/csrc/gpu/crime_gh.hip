@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -27,7 +28,8 @@ struct crime_gh {
   double *d_edges = nullptr;   // [n_nu+1]
   double *d_prefac = nullptr;  // [n_nu]
   int *d_guess = nullptr;      // channel guess table
-  void *d_amp = nullptr;       // (A, A psi) per |k|^2
+  void *d_amp = nullptr;       // amplitude per |k|^2
+  uint32_t *d_order = nullptr; // map-stage tiles by distance from the observer
   double *d_sub = nullptr;     // [3*n_sub]
   double *d_modes = nullptr;   // injected unit modes
   void *d_tw = nullptr;        // twiddles exp(+2 pi i m/N)
@@ -171,7 +173,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_guess, h->d_amp, h->d_sub,
+  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_guess, h->d_amp, h->d_order, h->d_sub,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -319,6 +321,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   f.unit_modes = h->d_modes;
   h->norm = pow(sqrt(2 * M_PI) / d->l_box, 3);  // fourier.c:403
   f.phi_scale = phi_balance(d);
+  f.psi_c = f.phi_scale * d->f0h0 / (f.dk * f.dk);
   HIP_OR_FAIL(h, hipMalloc(&h->d_amp, (size_t)amp_count(h->n) * 2 * esz));
   f.amp = h->d_amp;
   HIP_OR_FAIL(h, h->prec ? launch_amp_table<double>(f, h->d_amp, h->stream)
@@ -371,6 +374,32 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   m.n_guess = n_guess;
   m.g_lo = g_lo;
   m.g_inv = g_inv;
+
+  // tile order for the map stage: counting sort of the tiles by the distance of their centre from
+  // the observer, in bins of one cell
+  m.ntx = (h->n + TILE_X - 1) / TILE_X;
+  m.nty = (h->n + TILE_Y - 1) / TILE_Y;
+  m.ntz = (h->n + TILE_Z - 1) / TILE_Z;
+  {
+    const size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
+    const double dx = m.dx;
+    auto dist_bin = [&](size_t b) {
+      size_t tx = b % m.ntx, ty = (b / m.ntx) % m.nty, tz = b / ((size_t)m.ntx * m.nty);
+      double cx_ = dx * (tx * TILE_X + 0.5 * TILE_X) - d->pos_obs[0];
+      double cy_ = dx * (ty * TILE_Y + 0.5 * TILE_Y) - d->pos_obs[1];
+      double cz_ = dx * (tz * TILE_Z + 0.5 * TILE_Z) - d->pos_obs[2];
+      return (size_t)(sqrt(cx_ * cx_ + cy_ * cy_ + cz_ * cz_) / dx);
+    };
+    // distances inside the box are below sqrt(3) N cells; an observer outside it only clamps the order
+    const size_t nbins = (size_t)(2.0 * h->n) + 2;
+    std::vector<uint32_t> count(nbins + 1, 0), order(ntiles);
+    for (size_t b = 0; b < ntiles; b++) count[std::min(dist_bin(b), nbins - 1) + 1]++;
+    for (size_t i = 1; i <= nbins; i++) count[i] += count[i - 1];
+    for (size_t b = 0; b < ntiles; b++) order[count[std::min(dist_bin(b), nbins - 1)]++] = (uint32_t)b;
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_order, ntiles * sizeof(uint32_t)));
+    HIP_OR_FAIL(h, hipMemcpy(h->d_order, order.data(), ntiles * sizeof(uint32_t), hipMemcpyHostToDevice));
+    m.tile_order = h->d_order;
+  }
   m.nside = h->nside;
   m.npix = h->npix;
   m.sig = h->d_sig;

@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "gh_fft.hpp"
 
 namespace crime {
@@ -42,7 +44,19 @@ struct ColCfg {
     while (tx > 1 && tx * CB > budget) tx /= 2;
     return tx;
   }
-  static constexpr int TX = (tx_for(65536) >= 8 || tx_for(65536) == TXMAX) ? tx_for(65536) : tx_for(131072);
+  // 16 adjacent complex<float> columns = one 128-byte line per row: every HBM line fetched is used
+  // whole (8 columns would fetch each line twice).  Up to 128 KB of LDS: one workgroup of up to
+  // 1024 threads per CU.
+  static constexpr int TX = cmin(tx_for(131072), 1024 / TC);
+  static constexpr int NT = TX * TC;
+};
+
+// the fused fill + kz pass only writes (64-byte write requests waste nothing): a tile of at most
+// 64 KB keeps two workgroups per CU, so one's mode generation overlaps the other's FFT
+template <typename T, int N>
+struct FillCfg {
+  static constexpr int TC = ColCfg<T, N>::TC;
+  static constexpr int TX = cmin(ColCfg<T, N>::tx_for(65536), 1024 / TC);
   static constexpr int NT = TX * TC;
 };
 
@@ -72,8 +86,13 @@ struct FillParams {
   uint32_t seed;
   int mode_source;
   const double *unit_modes;
-  const void *amp;  // cx<T>[amp_count(n)]: (A, A psi) for k^2 = m dk^2, filled by launch_amp_table
+  const void *amp;  // AmpEntry<T>[amp_count(n)] for k^2 = m dk^2, filled by launch_amp_table
+  double psi_c;     // phi_scale f0 H0 / dk^2: psi = psi_c / m
 };
+
+// amplitude-table entry: A (FP32 grids) or (A, A psi) (FP64 grids)
+template <typename T>
+using AmpEntry = typename std::conditional<sizeof(T) == 8, cx<double>, float>::type;
 
 // entries of the amplitude table: every m = kx^2 + ky^2 + kz^2 with |k_i| <= N/2
 inline int amp_count(int n) { return 3 * (n / 2) * (n / 2) + 1; }
@@ -102,10 +121,13 @@ struct MapParams {
   const int *guess;        // [n_guess] channel of g_lo + j/g_inv (table mode)
   int n_guess;
   double g_lo, g_inv;
+  const uint32_t *tile_order;  // tiles by increasing distance of their centre from the observer
+  int ntx, nty, ntz;
 };
 
-// LDS aggregation of the scatter: one workgroup per TILE_X x TILE_Y x TILE_Z cells, AGG_SLOTS bins
-constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 4;
+// LDS aggregation of the scatter: one workgroup per TILE_X x TILE_Y x TILE_Z cells, AGG_SLOTS bins.
+// TILE_X = 16 cells of complex<float> = one 128-byte line per row of the grid.
+constexpr int TILE_X = 16, TILE_Y = 4, TILE_Z = 4;
 constexpr int AGG_SLOTS = 2048;
 
 template <typename T>

@@ -70,28 +70,30 @@ __device__ __forceinline__ cx<T> unit_mode(const FillParams &f, int kx, int ky, 
   }
 }
 
-// (A, A psi) for k^2 = m dk^2: sqrt(P(k)/dk^3 [e^{-r_s^2 k^2}]) (fourier.c:292-295) and the
-// velocity-potential factor psi = phi_scale f0 H0 / k^2 (fourier.c:298).  Built once per handle.
+// Amplitude per k^2 = m dk^2: A = sqrt(P(k)/dk^3 [e^{-r_s^2 k^2}]) (fourier.c:292-295) and, for
+// FP64 grids, also A psi with the velocity-potential factor psi = phi_scale f0 H0 / k^2
+// (fourier.c:298).  FP32 grids keep A alone (4 bytes per entry, 3 MB at N=1024, L2-sized) and form
+// psi = psi_c / m in registers.  Built once per handle.
 template <typename T>
-__global__ __launch_bounds__(256) void k_amp_table(FillParams f, cx<T> *__restrict__ tab, int mmax)
+__global__ __launch_bounds__(256) void k_amp_table(FillParams f, AmpEntry<T> *__restrict__ tab, int mmax)
 {
   int m = blockIdx.x * 256 + threadIdx.x;
   if (m > mmax) return;
-  if (m == 0) {
-    tab[0] = {(T)0, (T)0};
-    return;
+  double amp = 0, psi = 0;
+  if (m > 0) {
+    double k2 = (double)m * (f.dk * f.dk);
+    double sigma2 = pk_linear0(f, 0.5 * log10(k2)) * f.idk3;
+    if (f.do_smoothing) sigma2 *= exp(-f.r2_smooth * k2);
+    amp = sqrt(sigma2);
+    psi = f.phi_scale * (f.f0h0 / k2);
   }
-  double k2 = (double)m * (f.dk * f.dk);
-  double sigma2 = pk_linear0(f, 0.5 * log10(k2)) * f.idk3;
-  if (f.do_smoothing) sigma2 *= exp(-f.r2_smooth * k2);
-  double amp = sqrt(sigma2);
-  double psi = f.phi_scale * (f.f0h0 / k2);
-  tab[m] = {(T)amp, (T)(amp * psi)};
+  if constexpr (sizeof(T) == 8) tab[m] = {amp, amp * psi};
+  else tab[m] = (float)amp;
 }
 
 // F = delta + i phi_scale phi at full-grid position (kx, ky, kz), kx in [0, N)
 template <typename T>
-__device__ __forceinline__ cx<T> fill_element(const FillParams &f, const cx<T> *__restrict__ amp, int kx,
+__device__ __forceinline__ cx<T> fill_element(const FillParams &f, const AmpEntry<T> *__restrict__ amp, int kx,
                                               int ky, int kz)
 {
   const int n = f.n, h = n / 2;
@@ -100,7 +102,13 @@ __device__ __forceinline__ cx<T> fill_element(const FillParams &f, const cx<T> *
   int ax = kx <= h ? kx : n - kx;
   int ay = ky <= h ? ky : n - ky;
   int az = kz <= h ? kz : n - kz;
-  cx<T> ab = amp[ax * ax + ay * ay + az * az];
+  int m2 = ax * ax + ay * ay + az * az;
+  cx<T> ab;
+  if constexpr (sizeof(T) == 8) ab = amp[m2];
+  else {
+    float a = amp[m2];
+    ab = {a, a * ((float)f.psi_c / (float)m2)};
+  }
   int myn = (n - ky) % n, mzn = (n - kz) % n;  // -k in y and z
   cx<T> g;
   if (kx > 0 && kx < h) {
@@ -122,17 +130,17 @@ __device__ __forceinline__ cx<T> fill_element(const FillParams &f, const cx<T> *
 // kernels
 // ---------------------------------------------------------------------------------------------
 template <typename T, int N>
-__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_fill_z(cx<T> *__restrict__ F, FillParams f,
-                                                             const cx<T> *__restrict__ tw)
+__global__ __launch_bounds__((FillCfg<T, N>::NT)) void k_fill_z(cx<T> *__restrict__ F, FillParams f,
+                                                              const cx<T> *__restrict__ tw)
 {
-  using C = ColCfg<T, N>;
+  using C = FillCfg<T, N>;
   __shared__ cx<T> lds[N * C::TX];
   const int c = threadIdx.x % C::TX, t = threadIdx.x / C::TX;
   const int kx = blockIdx.x * C::TX + c, ky = blockIdx.y;
   const size_t n2 = (size_t)N * N;
   cx<T> *col = F + (size_t)kx + (size_t)N * ky;
   // the generator is too large to unroll into the register tile: stage the modes in LDS first
-  const cx<T> *amp = (const cx<T> *)f.amp;
+  const AmpEntry<T> *amp = (const AmpEntry<T> *)f.amp;
   for (int e = t; e < N; e += C::TC) lds[e * C::TX + c] = fill_element<T>(f, amp, kx, ky, e);
   __syncthreads();
   auto load = [&](int e) { return lds[e * C::TX + c]; };
@@ -229,10 +237,11 @@ static hipError_t launch_fields_n(cx<T> *F, const FillParams &f, const cx<T> *tw
                                   double *partials, double *sig, hipStream_t s, hipEvent_t *ev)
 {
   using CC = ColCfg<T, N>;
+  using FC = FillCfg<T, N>;
   using RC = RowCfg<T, N>;
   dim3 gcol(N / CC::TX, N);
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL((k_fill_z<T, N>), gcol, dim3(CC::NT), 0, s, F, f, tw);
+  hipLaunchKernelGGL((k_fill_z<T, N>), dim3(N / FC::TX, N), dim3(FC::NT), 0, s, F, f, tw);
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL((k_fft_y<T, N>), gcol, dim3(CC::NT), 0, s, F, tw);
   if (ev) (void)hipEventRecord(ev[2], s);
@@ -268,7 +277,7 @@ template <typename T>
 hipError_t launch_amp_table(const FillParams &f, void *tab, hipStream_t s)
 {
   int mmax = amp_count(f.n) - 1;
-  hipLaunchKernelGGL(k_amp_table<T>, dim3((mmax + 256) / 256), dim3(256), 0, s, f, (cx<T> *)tab, mmax);
+  hipLaunchKernelGGL(k_amp_table<T>, dim3((mmax + 256) / 256), dim3(256), 0, s, f, (AmpEntry<T> *)tab, mmax);
   return hipGetLastError();
 }
 template hipError_t launch_amp_table<float>(const FillParams &, void *, hipStream_t);

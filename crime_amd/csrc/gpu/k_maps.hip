@@ -140,14 +140,16 @@ __device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned 
 // global atomics scale with the bins a tile touches instead of with its samples.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T *__restrict__ maps,
-                                                  MapParams m, int ntx, int nty)
+                                                  MapParams m)
 {
   __shared__ unsigned long long keys[AGG_SLOTS];
   __shared__ T vals[AGG_SLOTS];
   const int n = m.n;
   const int t = threadIdx.x;
-  size_t b = blockIdx.x;
-  const int tx = (int)(b % ntx), ty = (int)((b / ntx) % nty), tz = (int)(b / ((size_t)ntx * nty));
+  // tiles run in order of distance from the observer, so the workgroups resident at one time cover a
+  // thin shell and their atomics fall on a few channels of the maps
+  const size_t b = m.tile_order[blockIdx.x];
+  const int tx = (int)(b % m.ntx), ty = (int)((b / m.ntx) % m.nty), tz = (int)(b / ((size_t)m.ntx * m.nty));
   const int ix = tx * TILE_X + (t % TILE_X), iy = ty * TILE_Y + ((t / TILE_X) % TILE_Y),
             iz = tz * TILE_Z + t / (TILE_X * TILE_Y);
   CellState s;
@@ -256,9 +258,8 @@ hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m
   if (zb > 65536u * 4) zb = 65536u * 4;
   hipLaunchKernelGGL(k_zero<T>, dim3(zb), dim3(256), 0, s, maps, nmaps);
   if (ev) (void)hipEventRecord(ev[1], s);
-  int ntx = (m.n + TILE_X - 1) / TILE_X, nty = (m.n + TILE_Y - 1) / TILE_Y, ntz = (m.n + TILE_Z - 1) / TILE_Z;
-  size_t ntiles = (size_t)ntx * nty * ntz;
-  hipLaunchKernelGGL(k_pixelize<T>, dim3((unsigned)ntiles), dim3(256), 0, s, F, maps, m, ntx, nty);
+  size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
+  hipLaunchKernelGGL(k_pixelize<T>, dim3((unsigned)ntiles), dim3(256), 0, s, F, maps, m);
   if (ev) (void)hipEventRecord(ev[2], s);
   return hipGetLastError();
 }
