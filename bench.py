@@ -35,6 +35,21 @@ def stage_bytes(n, prec):
     }
 
 
+KERNEL_OF_STAGE = {"fill_fft_z": "k_fill_z", "fft_y": "k_fft_y", "fft_x": "k_fft_x", "pixelize": "k_pixelize"}
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch from the committed PMC summary (profiles/hbm_traffic_<workload>.md)."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes":
+                out[f[1]] = float(f[2])
+    return out
+
+
 def cpu_baseline(workload, n_cpu, tmp):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import crime_amd
@@ -58,7 +73,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="production_1024", choices=WORKLOADS)
-    ap.add_argument("--cpu-n", type=int, default=512, help="grid of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-n", type=int, default=768, help="grid of the CPU-baseline sample (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (scale reference for PMC byte counters)")
     args = ap.parse_args()
@@ -140,7 +155,8 @@ def main():
             "config": {"workload": args.workload, "n_grid": n, "n_side": int(d.n_side), "n_nu": d.n_nu,
                        "samples_per_cell": d.n_sub, "l_box": d.l_box, "seed": d.seed},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(args.workload).get(KERNEL_OF_STAGE[dom]),
                          "bytes_per_launch": by[dom], "ms_per_launch": launch[dom]},
             "cpu_baseline": cpu,
             "stages": stages,
