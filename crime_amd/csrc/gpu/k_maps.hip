@@ -147,8 +147,11 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
   const int n = m.n;
   const int t = threadIdx.x;
   // tiles run in order of distance from the observer, so the workgroups resident at one time cover a
-  // thin shell and their atomics fall on a few channels of the maps
-  const size_t b = m.tile_order[blockIdx.x];
+  // thin shell and their atomics fall on a few channels of the maps.  2-D grid: a dispatch holds at
+  // most 2^32 work-items per dimension.
+  const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (bid >= (size_t)m.ntx * m.nty * m.ntz) return;
+  const size_t b = m.tile_order[bid];
   const int tx = (int)(b % m.ntx), ty = (int)((b / m.ntx) % m.nty), tz = (int)(b / ((size_t)m.ntx * m.nty));
   const int ix = tx * TILE_X + (t % TILE_X), iy = ty * TILE_Y + ((t / TILE_X) % TILE_Y),
             iz = tz * TILE_Z + t / (TILE_X * TILE_Y);
@@ -196,14 +199,14 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_cell_field(const cx<T> *__restrict__ F, MapParams m, size_t ncell,
                                                     int which, double *__restrict__ out)
 {
-  size_t cell = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (cell >= ncell) return;
-  int ix, iy, iz;
-  cell_coords<T>(m, cell, ix, iy, iz);
-  CellState s;
-  cell_velocity<T>(m, F, ix, iy, iz, s);
-  cell_hi(m, (double)F[cell].x, m.sig[0], s);
-  out[cell] = which == 2 ? s.vr : (which == 3 ? s.mass : s.dz);
+  for (size_t cell = (size_t)blockIdx.x * 256 + threadIdx.x; cell < ncell; cell += (size_t)gridDim.x * 256) {
+    int ix, iy, iz;
+    cell_coords<T>(m, cell, ix, iy, iz);
+    CellState s;
+    cell_velocity<T>(m, F, ix, iy, iz, s);
+    cell_hi(m, (double)F[cell].x, m.sig[0], s);
+    out[cell] = which == 2 ? s.vr : (which == 3 ? s.mass : s.dz);
+  }
 }
 
 template <typename T>
@@ -212,19 +215,19 @@ __global__ __launch_bounds__(256) void k_sample_indices(const cx<T> *__restrict_
                                                         int32_t *__restrict__ inu_out,
                                                         int64_t *__restrict__ ipix_out)
 {
-  int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncell) return;
-  size_t cell = (size_t)(cell0 + c);
-  int ix, iy, iz;
-  cell_coords<T>(m, cell, ix, iy, iz);
-  CellState s;
-  cell_velocity<T>(m, F, ix, iy, iz, s);
-  cell_hi(m, (double)F[cell].x, m.sig[0], s);
-  for (int isub = 0; isub < m.n_sub; isub++) {
-    int64_t ipix;
-    int inu = sample_bin(m, s, isub, ipix);
-    inu_out[c * m.n_sub + isub] = inu;
-    ipix_out[c * m.n_sub + isub] = ipix;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * 256) {
+    size_t cell = (size_t)(cell0 + c);
+    int ix, iy, iz;
+    cell_coords<T>(m, cell, ix, iy, iz);
+    CellState s;
+    cell_velocity<T>(m, F, ix, iy, iz, s);
+    cell_hi(m, (double)F[cell].x, m.sig[0], s);
+    for (int isub = 0; isub < m.n_sub; isub++) {
+      int64_t ipix;
+      int inu = sample_bin(m, s, isub, ipix);
+      inu_out[c * m.n_sub + isub] = inu;
+      ipix_out[c * m.n_sub + isub] = ipix;
+    }
   }
 }
 
@@ -232,34 +235,38 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_grid_get(const cx<T> *__restrict__ F, size_t n, int which,
                                                   double *__restrict__ out)
 {
-  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) out[i] = which == 0 ? (double)F[i].x : (double)F[i].y;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    out[i] = which == 0 ? (double)F[i].x : (double)F[i].y;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_grid_set(cx<T> *__restrict__ F, size_t n, int which,
                                                   const double *__restrict__ in)
 {
-  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  if (which == 0) F[i].x = (T)in[i];
-  else F[i].y = (T)in[i];
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    if (which == 0) F[i].x = (T)in[i];
+    else F[i].y = (T)in[i];
+  }
 }
 
-static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+// blocks for a grid-stride loop over n items: a dispatch holds at most 2^32 work-items per dimension
+static inline unsigned nblk(size_t n)
+{
+  size_t b = (n + 255) / 256;
+  return (unsigned)(b < (1u << 20) ? b : (1u << 20));
+}
 
 template <typename T>
 hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m, hipStream_t s,
                        hipEvent_t *ev)
 {
-  size_t ncell = (size_t)m.n * m.n * m.n;
   if (ev) (void)hipEventRecord(ev[0], s);
-  unsigned zb = (unsigned)((nmaps + 255) / 256);
-  if (zb > 65536u * 4) zb = 65536u * 4;
-  hipLaunchKernelGGL(k_zero<T>, dim3(zb), dim3(256), 0, s, maps, nmaps);
+  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(nmaps)), dim3(256), 0, s, maps, nmaps);
   if (ev) (void)hipEventRecord(ev[1], s);
   size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
-  hipLaunchKernelGGL(k_pixelize<T>, dim3((unsigned)ntiles), dim3(256), 0, s, F, maps, m);
+  unsigned gx = (unsigned)(ntiles < 65536 ? ntiles : 65536);
+  unsigned gy = (unsigned)((ntiles + gx - 1) / gx);
+  hipLaunchKernelGGL(k_pixelize<T>, dim3(gx, gy), dim3(256), 0, s, F, maps, m);
   if (ev) (void)hipEventRecord(ev[2], s);
   return hipGetLastError();
 }
