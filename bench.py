@@ -67,12 +67,80 @@ def cpu_baseline(workload, n_cpu, tmp):
                       f"Nside and samples per cell), FP64 C oracle with OpenMP, {dt:.1f} s"}
 
 
+def bench_ensemble(args):
+    """mocks/hour of the 96-member uneven ensemble (32 x 128^3/Nside 128, 32 x 192^3/Nside 256,
+    32 x 256^3/Nside 256; 64 channels; seeds 4000-4095; Omega_m, sigma_8 drawn per member)."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+
+    import crime_amd
+    import crime_amd.configs as cf
+
+    nmem = 96
+    with tempfile.TemporaryDirectory() as tmp:
+        inis = [cf.ensemble_member(tmp, i) for i in range(nmem)]
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(16) as ex:
+            pars = list(ex.map(crime_amd.Params, inis))
+        t_host = time.perf_counter() - t0
+        print(f"bench: host set-up of {nmem} members in {t_host:.2f} s (16 threads)", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        hs = [crime_amd.GetHI(p) for p in pars]
+        t_create = time.perf_counter() - t0
+        cells = sum(h.n ** 3 for h in hs)
+        for h in hs:
+            h.run()
+        # plain per-mock path on resident handles: one crime_gh_run (with its own sync) per member
+        t0 = time.perf_counter()
+        for h in hs:
+            h.run()
+        ms_plain = (time.perf_counter() - t0) * 1e3
+        print(f"bench: plain per-mock runs {ms_plain:.1f} ms for {nmem}", file=sys.stderr, flush=True)
+        b = crime_amd.Batch(hs, n_streams=args.streams)
+        for _ in range(args.warmup):
+            b.run()
+        ts = [b.run() for _ in range(args.steps)]
+        ms = sum(ts) / len(ts)
+        print(f"bench: batched graph {ms:.1f} ms per ensemble", file=sys.stderr, flush=True)
+        b.close()
+        # the drop-in sequence for one mock (what one GetHI run per mock does): host set-up, device
+        # set-up, run, maps to the host, release -- timed on 8 members of each kind
+        sample = [i for k in range(3) for i in range(32 * k, 32 * k + 8)]
+        for h in hs:
+            h.close()
+        t0 = time.perf_counter()
+        for i in sample:
+            p = crime_amd.Params(inis[i])
+            with crime_amd.GetHI(p) as g:
+                g.run()
+                g.maps()
+        ms_e2e = (time.perf_counter() - t0) * 1e3 / len(sample)
+    out = {
+        "metric": "mocks/hr batched 256^3",
+        "value": nmem * 3600e3 / ms,
+        "unit": "mocks/hour",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (Philox modes of BBKS P(k) per member cosmology; resident in HBM)",
+        "config": {"workload": "ensemble_96", "members": "32x128^3/Nside128 + 32x192^3/Nside256 + "
+                   "32x256^3/Nside256", "n_nu": 64, "cells_per_ensemble": cells, "streams": args.streams},
+        "plain_resident_mocks_per_hour": nmem * 3600e3 / ms_plain,
+        "plain_end_to_end_mocks_per_hour": 3600e3 / ms_e2e,
+        "speedup_vs_plain_resident": ms_plain / ms,
+        "speedup_vs_plain_end_to_end": (ms_e2e * nmem) / ms,
+        "cells_per_s": cells / (ms * 1e-3),
+        "host_setup_s_16_threads": t_host, "device_setup_s": t_create,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="production_1024", choices=WORKLOADS)
+    ap.add_argument("--workload", default="production_1024", choices=WORKLOADS + ("ensemble_96",))
+    ap.add_argument("--streams", type=int, default=4, help="concurrent streams of the ensemble graph")
     ap.add_argument("--cpu-n", type=int, default=768, help="grid of the CPU-baseline sample (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (scale reference for PMC byte counters)")
@@ -84,6 +152,9 @@ def main():
 
     import crime_amd
     import crime_amd.configs as cf
+
+    if args.workload == "ensemble_96":
+        return bench_ensemble(args)
 
     if args.calib_copy:
         import torch

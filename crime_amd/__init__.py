@@ -189,6 +189,43 @@ class GetHI:
         self.close()
 
 
+class Batch:
+    """Every member's field and map stages captured into one hipGraph (crime_gh_batch_*); each
+    run() replays it with a single launch.  Members keep their own maps and sigma^2 on the device."""
+
+    def __init__(self, handles, n_streams=4):
+        self._lib = _ffi.gpu_lib()
+        self.handles = list(handles)
+        arr = (C.c_void_p * len(self.handles))(*[h._h for h in self.handles])
+        b = C.c_void_p()
+        rc = self._lib.crime_gh_batch_create(arr, len(self.handles), n_streams, C.byref(b))
+        if rc:
+            raise CrimeError(rc, self._lib.crime_gh_batch_last_error(None).decode())
+        self._b = b
+
+    def run(self):
+        """Replay the graph; returns the wall time of the launch in ms."""
+        ms = C.c_double()
+        rc = self._lib.crime_gh_batch_run(self._b, C.byref(ms))
+        if rc:
+            raise CrimeError(rc, self._lib.crime_gh_batch_last_error(self._b).decode())
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._lib.crime_gh_batch_destroy(self._b)
+            self._b = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
 def run_batch(handles):
     """Run several handles through crime_gh_run_batch; returns one stats dict per handle."""
     lib = _ffi.gpu_lib()

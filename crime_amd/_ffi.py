@@ -142,6 +142,14 @@ def gpu_lib():
         lib.crime_gh_device_maps.argtypes = [H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
         lib.crime_gh_run_batch.restype = C.c_int
         lib.crime_gh_run_batch.argtypes = [C.POINTER(H), C.c_int, C.POINTER(CrimeGhStats)]
+        lib.crime_gh_batch_create.restype = C.c_int
+        lib.crime_gh_batch_create.argtypes = [C.POINTER(H), C.c_int, C.c_int, C.POINTER(H)]
+        lib.crime_gh_batch_run.restype = C.c_int
+        lib.crime_gh_batch_run.argtypes = [H, _dp]
+        lib.crime_gh_batch_destroy.restype = None
+        lib.crime_gh_batch_destroy.argtypes = [H]
+        lib.crime_gh_batch_last_error.restype = C.c_char_p
+        lib.crime_gh_batch_last_error.argtypes = [H]
         if lib.crime_gh_desc_size() != C.sizeof(CrimeGhDesc):
             raise ImportError("crime_gh_desc layout mismatch between libcrime_gh.so and crime_amd._ffi")
         _gpu = lib

@@ -83,7 +83,8 @@ def ensemble_member(outdir, i, **kw):
     rng = np.random.default_rng(4000 + i)
     om = float(rng.uniform(0.25, 0.35))
     s8 = float(rng.uniform(0.7, 0.9))
-    a = dict(n_grid=n_grid, n_side=n_side, n_nu=64, nu_lo=700.0, nu_hi=800.0, l_box=4800.0,
+    # l_box <= 0: the reference's own box, L = 2 r_max (1 + 2/N) for the member's cosmology
+    a = dict(n_grid=n_grid, n_side=n_side, n_nu=64, nu_lo=700.0, nu_hi=800.0, l_box=-1.0,
              seed=4000 + i, omega_m=om, sigma_8=s8, precision="single")
     a.update(kw)
     return make(outdir, f"member_{i:03d}", **a)

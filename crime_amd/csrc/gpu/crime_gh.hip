@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -433,27 +434,29 @@ static float ev_ms(hipEvent_t a, hipEvent_t b)
   return ms;
 }
 
-static int enqueue_fields(crime_gh *h, bool timed)
+static int enqueue_fields(crime_gh *h, bool timed, hipStream_t s = nullptr)
 {
   hipError_t e;
+  if (!s) s = h->stream;
   if (h->prec)
     e = launch_fields<double>(h->n, (cx<double> *)h->F, h->fill, (const cx<double> *)h->d_tw, h->norm,
-                              h->d_partials, h->d_sig, h->stream, timed ? h->ev : nullptr);
+                              h->d_partials, h->d_sig, s, timed ? h->ev : nullptr);
   else
     e = launch_fields<float>(h->n, (cx<float> *)h->F, h->fill, (const cx<float> *)h->d_tw, h->norm,
-                             h->d_partials, h->d_sig, h->stream, timed ? h->ev : nullptr);
+                             h->d_partials, h->d_sig, s, timed ? h->ev : nullptr);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "field kernels: %s", hipGetErrorString(e));
   return 0;
 }
 
-static int enqueue_maps(crime_gh *h, bool timed)
+static int enqueue_maps(crime_gh *h, bool timed, hipStream_t s = nullptr)
 {
   hipError_t e;
+  if (!s) s = h->stream;
   if (h->prec)
-    e = launch_maps<double>((const cx<double> *)h->F, (double *)h->maps, h->nmaps, h->map, h->stream,
+    e = launch_maps<double>((const cx<double> *)h->F, (double *)h->maps, h->nmaps, h->map, s,
                             timed ? h->ev + 5 : nullptr);
   else
-    e = launch_maps<float>((const cx<float> *)h->F, (float *)h->maps, h->nmaps, h->map, h->stream,
+    e = launch_maps<float>((const cx<float> *)h->F, (float *)h->maps, h->nmaps, h->map, s,
                            timed ? h->ev + 5 : nullptr);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "map kernels: %s", hipGetErrorString(e));
   return 0;
@@ -637,12 +640,152 @@ extern "C" int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes)
   return 0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// batches: one captured hipGraph over many handles
+// ---------------------------------------------------------------------------------------------
+struct crime_gh_batch {
+  std::vector<crime_gh *> hs;
+  int device = 0;
+  hipStream_t origin = nullptr;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> evs;  // fork + one join per stream
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+};
+
+static int batch_error(crime_gh_batch *b, int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  else g_create_err = buf;
+  return code;
+}
+
+#define BATCH_HIP(b, call)                                                                           \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess) return batch_error(b, CRIME_GH_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+static void batch_free(crime_gh_batch *b)
+{
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto e : b->evs) (void)hipEventDestroy(e);
+  for (auto s : b->streams) (void)hipStreamDestroy(s);
+  if (b->origin) (void)hipStreamDestroy(b->origin);
+}
+
+static int batch_capture(crime_gh_batch *b, int n_streams)
+{
+  BATCH_HIP(b, hipSetDevice(b->device));
+  BATCH_HIP(b, hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking));
+  b->streams.resize(n_streams, nullptr);
+  b->evs.resize(n_streams + 1, nullptr);
+  for (auto &s : b->streams) BATCH_HIP(b, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  for (auto &e : b->evs) BATCH_HIP(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  BATCH_HIP(b, hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  hipError_t e = hipEventRecord(b->evs[0], b->origin);
+  for (int i = 0; i < n_streams && e == hipSuccess; i++) e = hipStreamWaitEvent(b->streams[i], b->evs[0], 0);
+  // members round-robin over the streams, largest grids first so the tail is made of small members
+  std::vector<int> order(b->hs.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->hs[a]->ncell > b->hs[c]->ncell; });
+  for (size_t k = 0; k < order.size() && e == hipSuccess && !rc; k++) {
+    crime_gh *h = b->hs[order[k]];
+    hipStream_t s = b->streams[k % n_streams];
+    rc = enqueue_fields(h, false, s);
+    if (!rc) rc = enqueue_maps(h, false, s);
+    if (rc) batch_error(b, rc, "member %d: %s", order[k], h->err.c_str());
+  }
+  for (int i = 0; i < n_streams && e == hipSuccess; i++) {
+    e = hipEventRecord(b->evs[i + 1], b->streams[i]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->origin, b->evs[i + 1], 0);
+  }
+  hipError_t e2 = hipStreamEndCapture(b->origin, &b->graph);
+  if (rc) return rc;
+  if (e != hipSuccess) return batch_error(b, CRIME_GH_EHIP, "capture: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return batch_error(b, CRIME_GH_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+  BATCH_HIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+extern "C" int crime_gh_batch_create(crime_gh_t **hs, int n, int n_streams, crime_gh_batch_t **out)
+{
+  if (!out) return batch_error(nullptr, CRIME_GH_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!hs || n < 1) return batch_error(nullptr, CRIME_GH_EINVAL, "empty batch");
+  if (n_streams < 1 || n_streams > 32) return batch_error(nullptr, CRIME_GH_EINVAL, "n_streams=%d", n_streams);
+  for (int i = 0; i < n; i++) {
+    if (!hs[i]) return batch_error(nullptr, CRIME_GH_EINVAL, "member %d is NULL", i);
+    if (hs[i]->d.device != hs[0]->d.device)
+      return batch_error(nullptr, CRIME_GH_EINVAL, "member %d is on device %d, member 0 on %d", i,
+                         hs[i]->d.device, hs[0]->d.device);
+    for (int j = 0; j < i; j++)
+      if (hs[j] == hs[i]) return batch_error(nullptr, CRIME_GH_EINVAL, "member %d repeats member %d", i, j);
+  }
+  crime_gh_batch *b = new crime_gh_batch();
+  b->hs.assign(hs, hs + n);
+  b->device = hs[0]->d.device;
+  int rc = batch_capture(b, n_streams < n ? n_streams : n);
+  if (rc) {
+    g_create_err = b->err;
+    batch_free(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int crime_gh_batch_run(crime_gh_batch_t *b, double *ms)
+{
+  if (!b) return batch_error(nullptr, CRIME_GH_EINVAL, "batch is NULL");
+  BATCH_HIP(b, hipSetDevice(b->device));
+  auto t0 = std::chrono::steady_clock::now();
+  BATCH_HIP(b, hipGraphLaunch(b->exec, b->origin));
+  BATCH_HIP(b, hipStreamSynchronize(b->origin));
+  if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (auto h : b->hs) h->fields_done = true;
+  return 0;
+}
+
+extern "C" void crime_gh_batch_destroy(crime_gh_batch_t *b)
+{
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->origin) (void)hipStreamSynchronize(b->origin);
+  batch_free(b);
+  delete b;
+}
+
+extern "C" const char *crime_gh_batch_last_error(const crime_gh_batch_t *b)
+{
+  return b ? b->err.c_str() : g_create_err.c_str();
+}
+
 extern "C" int crime_gh_run_batch(crime_gh_t **hs, int n, crime_gh_stats *st)
 {
-  if (!hs || n < 1) return set_error(nullptr, CRIME_GH_EINVAL, "empty batch");
-  for (int i = 0; i < n; i++) {
-    int rc = crime_gh_run(hs[i], st ? st + i : nullptr);
-    if (rc) return rc;
+  crime_gh_batch_t *b = nullptr;
+  int rc = crime_gh_batch_create(hs, n, 4, &b);
+  if (rc) return rc;
+  double ms = 0;
+  rc = crime_gh_batch_run(b, &ms);
+  if (!rc && st) {
+    for (int i = 0; i < n; i++) {
+      memset(&st[i], 0, sizeof(st[i]));
+      st[i].ms_total = ms;
+      rc = crime_gh_get_sigma2(hs[i], &st[i].sigma2_gauss, &st[i].mean_gauss);
+      if (rc) break;
+    }
   }
-  return 0;
+  if (rc) g_create_err = b->err.empty() ? std::string(crime_gh_last_error(nullptr)) : b->err;
+  crime_gh_batch_destroy(b);
+  return rc;
 }

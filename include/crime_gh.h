@@ -179,9 +179,20 @@ int crime_gh_sample_indices(crime_gh_t *h, int64_t cell0, int64_t ncell, int32_t
  * for callers that hand the maps to further device work without a host round trip. */
 int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes);
 
-/* Batched run: every handle's fields and maps in one captured hipGraph launch on hs[0]'s device.
- * Handles may differ in N, Nside, channels, cosmology and seed.  No counterpart in the reference
- * (which runs one realisation per process); st may be NULL or an array of n. */
+/* Batched runs (no counterpart in the reference, which runs one realisation per process).
+ * A batch captures every member's field and map stages into ONE hipGraph, members forked over
+ * n_streams concurrent streams; crime_gh_batch_run replays it with one launch.  Members may differ in
+ * N, Nside, channels, cosmology and seed; they must live on one device and appear once each.  The
+ * graph clears the maps with a kernel (no memset nodes).  Each member's sigma^2 stays on the device
+ * (crime_gh_get_sigma2). */
+typedef struct crime_gh_batch crime_gh_batch_t;
+int crime_gh_batch_create(crime_gh_t **hs, int n, int n_streams, crime_gh_batch_t **out);
+int crime_gh_batch_run(crime_gh_batch_t *b, double *ms); /* ms: wall time of the launch, may be NULL */
+void crime_gh_batch_destroy(crime_gh_batch_t *b);
+const char *crime_gh_batch_last_error(const crime_gh_batch_t *b);
+
+/* One-shot convenience: batch_create (4 streams) + batch_run + batch_destroy.  st may be NULL or an
+ * array of n; only sigma2_gauss/mean_gauss and ms_total (the whole launch) are filled. */
 int crime_gh_run_batch(crime_gh_t **hs, int n, crime_gh_stats *st);
 
 #ifdef __cplusplus
