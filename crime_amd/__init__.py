@@ -169,6 +169,19 @@ class GetHI:
             ipix.ctypes.data_as(C.POINTER(C.c_int64))))
         return inu, ipix
 
+    def power_spectrum(self, nbins=None, bin_dk=None):
+        """Shell-averaged P(k) of the resident Gaussian delta: (k_mean [h/Mpc], P [(Mpc/h)^3], n_modes).
+        Default shells are one fundamental mode wide up to the Nyquist k.  Consumes the fields."""
+        dk = 2 * np.pi / self.desc.l_box
+        bin_dk = dk if bin_dk is None else bin_dk
+        nbins = int(np.ceil((self.n // 2) * dk / bin_dk)) if nbins is None else nbins
+        k = np.empty(nbins)
+        pk = np.empty(nbins)
+        cnt = np.empty(nbins, np.int64)
+        self._check(self._lib.crime_gh_power_spectrum(self._h, nbins, bin_dk, _dptr(k), _dptr(pk),
+                                                      cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+        return k, pk, cnt
+
     def device_maps(self):
         p, nb = C.c_void_p(), C.c_size_t()
         self._check(self._lib.crime_gh_device_maps(self._h, C.byref(p), C.byref(nb)))

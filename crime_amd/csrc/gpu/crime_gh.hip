@@ -520,6 +520,42 @@ extern "C" int crime_gh_run(crime_gh_t *h, crime_gh_stats *st)
   return 0;
 }
 
+extern "C" int crime_gh_power_spectrum(crime_gh_t *h, int nbins, double bin_dk, double *k_mean, double *pk,
+                                       int64_t *nmodes)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (!k_mean || !pk || !nmodes) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (nbins < 1 || nbins > PK_MAX_BINS) return set_error(h, CRIME_GH_EINVAL, "nbins %d not in [1, %d]", nbins, PK_MAX_BINS);
+  if (!(bin_dk > 0)) return set_error(h, CRIME_GH_EINVAL, "bin_dk must be > 0");
+  if (!h->fields_done) return set_error(h, CRIME_GH_ESTATE, "power_spectrum before create_d_and_vr_fields");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  const double dk = 2 * M_PI / h->d.l_box, n3 = (double)h->n * h->n * h->n;
+  double *acc = nullptr;
+  HIP_OR_FAIL(h, hipMalloc((void **)&acc, 3 * (size_t)nbins * sizeof(double)));
+  // acc is cleared by a copy (no memset node) before the transform
+  std::vector<double> hacc(3 * (size_t)nbins, 0.0);
+  hipError_t e = hipMemcpyAsync(acc, hacc.data(), hacc.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  const double pscale = h->d.l_box * h->d.l_box * h->d.l_box / (n3 * n3), binw = bin_dk / dk;
+  if (e == hipSuccess)
+    e = h->prec ? launch_power_spectrum<double>(h->n, (cx<double> *)h->F, (const cx<double> *)h->d_tw, pscale, binw,
+                                                nbins, acc, h->stream)
+                : launch_power_spectrum<float>(h->n, (cx<float> *)h->F, (const cx<float> *)h->d_tw, pscale, binw,
+                                               nbins, acc, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hacc.data(), acc, hacc.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(acc);
+  h->fields_done = false;  // the grid now holds the transform
+  if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "power_spectrum: %s", hipGetErrorString(e));
+  for (int i = 0; i < nbins; i++) {
+    double c = hacc[3 * i + 2];
+    nmodes[i] = (int64_t)c;
+    k_mean[i] = c > 0 ? hacc[3 * i + 1] / c * dk : 0.0;
+    pk[i] = c > 0 ? hacc[3 * i] / c : 0.0;
+  }
+  return 0;
+}
+
 extern "C" int crime_gh_copy_maps(crime_gh_t *h, float *out)
 {
   if (!h || !out) return set_error(h, CRIME_GH_EINVAL, "NULL argument");

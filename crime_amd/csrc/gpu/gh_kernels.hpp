@@ -133,6 +133,11 @@ constexpr int AGG_SLOTS = 2048;
 template <typename T>
 hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
                          double *partials, double *sig, hipStream_t s, hipEvent_t *ev);
+
+constexpr int PK_MAX_BINS = 2048;
+template <typename T>
+hipError_t launch_power_spectrum(int n, cx<T> *F, const cx<T> *tw, double pscale, double binw, int nbins,
+                                 double *acc, hipStream_t s);
 int fields_partials_count(int n, int prec);
 
 template <typename T>

@@ -203,6 +203,101 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_fft_x(cx<T> *__restrict_
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// P(k) estimator: forward DFT of the resident grid as conj(IDFT(conj F)) -- kx pass on conj(F),
+// then the ky and kz passes -- followed by shell binning of |delta_k|^2.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_fft_x_conj(cx<T> *__restrict__ F,
+                                                                 const cx<T> *__restrict__ tw)
+{
+  using C = RowCfg<T, N>;
+  __shared__ cx<T> lds[N * C::TY];
+  const int t = threadIdx.x % C::TC, b = threadIdx.x / C::TC;
+  cx<T> *line = F + ((size_t)blockIdx.x * C::TY + b) * N;
+  auto load = [&](int e) { return cconj(line[e]); };
+  auto store = [&](int e, cx<T> v) { line[e] = v; };
+  auto slot = [&](int e) -> cx<T> & { return lds[b * N + e]; };
+  Stockham<T, N, C::TC>::run(t, load, store, slot, tw);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_fft_z(cx<T> *__restrict__ F, const cx<T> *__restrict__ tw)
+{
+  using C = ColCfg<T, N>;
+  __shared__ cx<T> lds[N * C::TX];
+  const int c = threadIdx.x % C::TX, t = threadIdx.x / C::TX;
+  const int kx = blockIdx.x * C::TX + c, ky = blockIdx.y;
+  const size_t n2 = (size_t)N * N;
+  cx<T> *col = F + (size_t)kx + (size_t)N * ky;
+  auto load = [&](int e) { return col[(size_t)e * n2]; };
+  auto store = [&](int e, cx<T> v) { col[(size_t)e * n2] = v; };
+  auto slot = [&](int e) -> cx<T> & { return lds[e * C::TX + c]; };
+  Stockham<T, N, C::TC>::run(t, load, store, slot, tw);
+}
+
+// G = IDFT(conj F) = conj(DFT F).  delta real => DFT(delta)_k = (DFT F_k + conj(DFT F_{-k}))/2
+// = (conj G_k + G_{-k})/2; P_est = |DFT(delta)_k|^2 L^3 / N^6 summed into |k|/dk shells of width binw.
+// acc[3*i] += P, acc[3*i+1] += |k|/dk, acc[3*i+2] += 1 (every k of the full grid).
+template <typename T>
+__global__ __launch_bounds__(256) void k_pk_bin(const cx<T> *__restrict__ G, int n, double pscale, double binw,
+                                                int nbins, double *__restrict__ acc)
+{
+  __shared__ double h[3 * PK_MAX_BINS];
+  for (int i = threadIdx.x; i < 3 * nbins; i += 256) h[i] = 0;
+  __syncthreads();
+  const size_t nn = (size_t)n, n3 = nn * nn * nn;
+  const int hf = n / 2;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n3; c += (size_t)gridDim.x * 256) {
+    int kx = (int)(c % nn), ky = (int)((c / nn) % nn), kz = (int)(c / (nn * nn));
+    int ax = kx <= hf ? kx : n - kx, ay = ky <= hf ? ky : n - ky, az = kz <= hf ? kz : n - kz;
+    double kk = sqrt((double)(ax * ax + ay * ay + az * az));
+    int bin = (int)(kk / binw);
+    if (bin >= nbins) continue;
+    size_t cm = (size_t)((n - kx) % n) + nn * ((size_t)((n - ky) % n) + nn * (size_t)((n - kz) % n));
+    cx<T> g = G[c], gm = G[cm];
+    double re = 0.5 * ((double)g.x + (double)gm.x), im = 0.5 * ((double)gm.y - (double)g.y);
+    atomicAdd(&h[3 * bin], (re * re + im * im) * pscale);
+    atomicAdd(&h[3 * bin + 1], kk);
+    atomicAdd(&h[3 * bin + 2], 1.0);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * nbins; i += 256)
+    if (h[i] != 0) atomicAdd(&acc[i], h[i]);
+}
+
+template <typename T, int N>
+static hipError_t launch_pk_n(cx<T> *F, const cx<T> *tw, double pscale, double binw, int nbins, double *acc,
+                              hipStream_t s)
+{
+  using CC = ColCfg<T, N>;
+  using RC = RowCfg<T, N>;
+  hipLaunchKernelGGL((k_fft_x_conj<T, N>), dim3((N * N) / RC::TY), dim3(RC::NT), 0, s, F, tw);
+  hipLaunchKernelGGL((k_fft_y<T, N>), dim3(N / CC::TX, N), dim3(CC::NT), 0, s, F, tw);
+  hipLaunchKernelGGL((k_fft_z<T, N>), dim3(N / CC::TX, N), dim3(CC::NT), 0, s, F, tw);
+  size_t n3 = (size_t)N * N * N;
+  unsigned nb = (unsigned)((n3 + 255) / 256 < 8192 ? (n3 + 255) / 256 : 8192);
+  hipLaunchKernelGGL(k_pk_bin<T>, dim3(nb), dim3(256), 0, s, F, N, pscale, binw, nbins, acc);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_power_spectrum(int n, cx<T> *F, const cx<T> *tw, double pscale, double binw, int nbins,
+                                 double *acc, hipStream_t s)
+{
+  switch (n) {
+#define CRIME_CASE(NN) \
+  case NN: return launch_pk_n<T, NN>(F, tw, pscale, binw, nbins, acc, s);
+    CRIME_FOR_EACH_N(CRIME_CASE)
+#undef CRIME_CASE
+  default: return hipErrorInvalidValue;
+  }
+}
+template hipError_t launch_power_spectrum<float>(int, cx<float> *, const cx<float> *, double, double, int,
+                                                 double *, hipStream_t);
+template hipError_t launch_power_spectrum<double>(int, cx<double> *, const cx<double> *, double, double, int,
+                                                  double *, hipStream_t);
+
 __global__ __launch_bounds__(1024) void k_sigma_finalize(const double *__restrict__ partials, int nparts,
                                                          double ncells, double *__restrict__ out)
 {

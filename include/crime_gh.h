@@ -175,6 +175,13 @@ int crime_gh_set_sigma2(crime_gh_t *h, double sigma2, double mean);
 int crime_gh_sample_indices(crime_gh_t *h, int64_t cell0, int64_t ncell, int32_t *inu,
                             int64_t *ipix);
 
+/* Power spectrum of the resident Gaussian field delta (no counterpart in the reference): a forward
+ * 3-D FFT of the grid and |delta_k|^2 L^3/N^6 averaged over shells |k| in [i, i+1) * bin_dk (h/Mpc),
+ * i < nbins <= 2048.  Outputs per shell: mean |k| (h/Mpc), mean P (Mpc/h)^3 and the number of grid
+ * modes (k and -k both counted).  The transform runs in place: the fields are consumed, and
+ * crime_gh_create_d_and_vr_fields must run again before crime_gh_mk_T_maps. */
+int crime_gh_power_spectrum(crime_gh_t *h, int nbins, double bin_dk, double *k_mean, double *pk, int64_t *nmodes);
+
 /* Device pointer and size of the resident maps (n_nu*12*Nside^2 elements of the grid precision),
  * for callers that hand the maps to further device work without a host round trip. */
 int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes);
