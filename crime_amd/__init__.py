@@ -20,7 +20,8 @@ from . import _ffi
 from ._ffi import (BIAS_CONST, BIAS_LINEAR, BIAS_REFERENCE, FP32, FP64, GRID_DENS, GRID_DZRSD, GRID_MASS,
                    GRID_RVEL, GRID_VPOT, MODES_INJECTED, MODES_PHILOX, NU_TABLE, NU_UNIFORM, CrimeGhDesc)
 
-__all__ = ["Params", "GetHI", "CrimeError", "write_ini", "write_bbks_pk", "FP32", "FP64",
+__all__ = ["Params", "GetHI", "Sht", "Batch", "run_batch", "alm_index", "CrimeError", "write_ini",
+           "write_bbks_pk", "FP32", "FP64",
            "GRID_DENS", "GRID_VPOT", "GRID_RVEL", "GRID_MASS", "GRID_DZRSD", "MODES_PHILOX",
            "MODES_INJECTED", "NU_TABLE", "NU_UNIFORM", "BIAS_REFERENCE", "BIAS_CONST", "BIAS_LINEAR"]
 
@@ -249,3 +250,86 @@ def run_batch(handles):
     if rc:
         raise CrimeError(rc, lib.crime_gh_last_error(None).decode())
     return [s.as_dict() for s in st]
+
+
+def alm_index(l, m, lmax):
+    """he_indexlm (src/healpix_extra.c:36-42): position of a_lm, m >= 0, in a map's alm array."""
+    return l + m * lmax - m * (m - 1) // 2 if m > 0 else l
+
+
+class Sht:
+    """HEALPix analysis plan of libcrime_gh.so (include/crime_sht.h): map2alm as he_map2alm
+    (src/healpix_extra.c:88-130) does it, and angular power spectra, in FP64 on the device."""
+
+    def __init__(self, nside, lmax=None, max_maps=1, device=0):
+        self._lib = _ffi.gpu_lib()
+        self.nside = int(nside)
+        self.lmax = 3 * self.nside - 1 if lmax is None else int(lmax)
+        self.max_maps = int(max_maps)
+        self.npix = 12 * self.nside ** 2
+        self.nalm = (self.lmax + 1) * (self.lmax + 2) // 2
+        h = C.c_void_p()
+        rc = self._lib.crime_sht_create(self.nside, self.lmax, self.max_maps, device, C.byref(h))
+        if rc:
+            raise CrimeError(rc, self._lib.crime_sht_last_error(None).decode())
+        self._p = h
+
+    def _check(self, rc):
+        if rc:
+            raise CrimeError(rc, self._lib.crime_sht_last_error(self._p).decode())
+
+    def map2alm(self, maps):
+        """maps (nmaps, Npix) RING on the host -> alms (nmaps, nalm) complex, he_indexlm order."""
+        m = np.ascontiguousarray(np.atleast_2d(maps), dtype=np.float64)
+        assert m.shape[1] == self.npix
+        out = np.empty((m.shape[0], self.nalm), np.complex128)
+        self._check(self._lib.crime_sht_map2alm(self._p, m.shape[0], _dptr(m), out.ctypes.data_as(_ffi._dp)))
+        return out
+
+    def map2alm_device(self, dptr, nmaps, fp64):
+        out = np.empty((nmaps, self.nalm), np.complex128)
+        self._check(self._lib.crime_sht_map2alm_device(self._p, nmaps, C.c_void_p(dptr), int(fp64),
+                                                       out.ctypes.data_as(_ffi._dp)))
+        return out
+
+    def anafast(self, maps, cross=True):
+        """C_l of host maps (nmaps, Npix): (nmaps, nmaps, lmax+1) if cross else (nmaps, lmax+1)."""
+        m = np.ascontiguousarray(np.atleast_2d(maps), dtype=np.float64)
+        assert m.shape[1] == self.npix
+        n = m.shape[0]
+        out = np.empty((n, n, self.lmax + 1) if cross else (n, self.lmax + 1))
+        self._check(self._lib.crime_sht_anafast(self._p, n, _dptr(m), int(cross), _dptr(out)))
+        return out
+
+    def anafast_device(self, dptr, nmaps, fp64, cross=True):
+        """C_l of device maps: (nmaps, nmaps, lmax+1) if cross else (nmaps, lmax+1)."""
+        shape = (nmaps, nmaps, self.lmax + 1) if cross else (nmaps, self.lmax + 1)
+        out = np.empty(shape)
+        self._check(self._lib.crime_sht_anafast_device(self._p, nmaps, C.c_void_p(dptr), int(fp64), int(cross),
+                                                       _dptr(out)))
+        return out
+
+    def cls_of(self, gh, cross=True, channels=None):
+        """C_l of a GetHI handle's resident maps (all channels, or the first `channels`)."""
+        p, _ = gh.device_maps()
+        n = gh.n_nu if channels is None else int(channels)
+        return self.anafast_device(p, n, gh.precision == FP64, cross)
+
+    def stats(self):
+        st = _ffi.CrimeShtStats()
+        self._check(self._lib.crime_sht_get_stats(self._p, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._lib.crime_sht_destroy(self._p)
+            self._p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()

@@ -73,6 +73,16 @@ class CrimeGhDesc(C.Structure):
     ]
 
 
+class CrimeShtStats(C.Structure):
+    """Mirror of ``crime_sht_stats``."""
+
+    _fields_ = [("ms_rings", C.c_double), ("ms_legendre", C.c_double), ("ms_alm", C.c_double),
+                ("ms_cls", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class CrimeGhStats(C.Structure):
     """Mirror of ``crime_gh_stats``."""
 
@@ -152,6 +162,22 @@ def gpu_lib():
         lib.crime_gh_batch_destroy.argtypes = [H]
         lib.crime_gh_batch_last_error.restype = C.c_char_p
         lib.crime_gh_batch_last_error.argtypes = [H]
+        lib.crime_sht_create.restype = C.c_int
+        lib.crime_sht_create.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(H)]
+        lib.crime_sht_destroy.restype = None
+        lib.crime_sht_destroy.argtypes = [H]
+        lib.crime_sht_last_error.restype = C.c_char_p
+        lib.crime_sht_last_error.argtypes = [H]
+        lib.crime_sht_map2alm.restype = C.c_int
+        lib.crime_sht_map2alm.argtypes = [H, C.c_int, _dp, _dp]
+        lib.crime_sht_map2alm_device.restype = C.c_int
+        lib.crime_sht_map2alm_device.argtypes = [H, C.c_int, C.c_void_p, C.c_int, _dp]
+        lib.crime_sht_anafast_device.restype = C.c_int
+        lib.crime_sht_anafast_device.argtypes = [H, C.c_int, C.c_void_p, C.c_int, C.c_int, _dp]
+        lib.crime_sht_anafast.restype = C.c_int
+        lib.crime_sht_anafast.argtypes = [H, C.c_int, _dp, C.c_int, _dp]
+        lib.crime_sht_get_stats.restype = C.c_int
+        lib.crime_sht_get_stats.argtypes = [H, C.POINTER(CrimeShtStats)]
         if lib.crime_gh_desc_size() != C.sizeof(CrimeGhDesc):
             raise ImportError("crime_gh_desc layout mismatch between libcrime_gh.so and crime_amd._ffi")
         _gpu = lib

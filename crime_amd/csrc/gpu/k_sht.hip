@@ -1,0 +1,382 @@
+// HEALPix analysis on gfx950: ring spectra, Legendre table, alm GEMMs (FP64 MFMA) and C_l.
+//
+// a_lm = w sum_rings lambda_lm(z_r) F_m(r),  F_m(r) = sum_j f(r, j) e^{-i m phi_j},  w = 4 pi / Npix,
+// the sum of he_map2alm (src/healpix_extra.c:88-130, libsharp with unit ring weights) split the way
+// every fast SHT splits it (Reinecke 2011, A&A 526, A108): an FFT-like pass per ring, then per m a
+// real (l x ring) by complex (ring x map) product.  North/south ring pairs are folded first:
+// lambda_lm(-z) = (-1)^{l+m} lambda_lm(z), so rows with l - m even use E = F_north + F_south and rows
+// with l - m odd use O = F_north - F_south over the 2 Nside northern rings (equator included once).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "gh_sht.hpp"
+
+namespace crime {
+
+// ---------------------------------------------------------------------------------------------
+// ring spectra: one workgroup per (map, northern ring); the mirror ring is transformed alongside.
+// DFT of length n = P Q (P a power of two): S[j1][k1] = sum_j2 f[P j2 + j1] w^{P j2 k1},
+// D[k] = sum_j1 w^{j1 k} S[j1][k mod Q], w = e^{-2 pi i / n}; F_m = e^{-i pi m s / n} D[m mod n].
+// LDS: twiddles (n complex), f (n real), S (n complex), D north and south (K complex each).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int sht_pick_p(int n, int K)
+{
+  int best = 1, p = 1;
+  double bestc = 1e300;
+  while (n % p == 0 && p <= n) {
+    int q = n / p;
+    double c = 2.0 * n * q + 4.0 * K * p;
+    if (c < bestc) { bestc = c; best = p; }
+    p *= 2;
+  }
+  return best;
+}
+
+template <typename TM>
+__device__ void ring_spectrum(const TM *__restrict__ f_g, int n, int K, int P, const double2 *tw, double *f,
+                              double2 *S, double2 *D)
+{
+  const int Q = n / P;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) f[j] = (double)f_g[j];
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+    int j1 = idx / Q, k1 = idx - j1 * Q;
+    double re = 0, im = 0;
+    int t = 0;                   // (P j2 k1) mod n
+    const int step = P * k1 % n;
+    for (int j2 = 0; j2 < Q; j2++) {
+      double v = f[P * j2 + j1];
+      double2 w = tw[t];
+      re = fma(v, w.x, re);
+      im = fma(v, w.y, im);
+      t += step;
+      if (t >= n) t -= n;
+    }
+    S[idx] = make_double2(re, im);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    const int k1 = k % Q;
+    double re = 0, im = 0;
+    int t = 0;                   // (j1 k) mod n
+    const int step = k % n;
+    for (int j1 = 0; j1 < P; j1++) {
+      double2 s = S[j1 * Q + k1];
+      double2 w = tw[t];
+      re = fma(s.x, w.x, fma(-s.y, w.y, re));
+      im = fma(s.x, w.y, fma(s.y, w.x, im));
+      t += step;
+      if (t >= n) t -= n;
+    }
+    D[k] = make_double2(re, im);
+  }
+  __syncthreads();
+}
+
+template <typename TM>
+__global__ __launch_bounds__(512) void k_ring_dft(const TM *__restrict__ maps, int64_t npix,
+                                                  const ShtRing *__restrict__ rings, int R, int RP, int lmax,
+                                                  int ldg, double w, double *__restrict__ G)
+{
+  extern __shared__ double2 lds_sht[];
+  const int c = blockIdx.x, r = blockIdx.y;
+  const ShtRing ri = rings[r];
+  const int n = ri.nphi;
+  const int K = n < lmax + 1 ? n : lmax + 1;
+  const int P = sht_pick_p(n, K);
+  double2 *tw = lds_sht;                         // n
+  double2 *S = tw + n;                           // n
+  double2 *Dn = S + n;                           // K
+  double2 *Ds = Dn + K;                          // K
+  double *f = (double *)(Ds + K);                // n
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    double sn, cs;
+    sincospi(-2.0 * (double)t / (double)n, &sn, &cs);
+    tw[t] = make_double2(cs, sn);
+  }
+  const TM *mp = maps + (size_t)c * npix;
+  ring_spectrum(mp + ri.first_n, n, K, P, tw, f, S, Dn);
+  const bool eq = (r == R - 1);
+  if (!eq) ring_spectrum(mp + ri.first_s, n, K, P, tw, f, S, Ds);
+  const size_t plane = (size_t)RP * ldg;         // one (m, parity) slice of G
+  for (int m = threadIdx.x; m <= lmax; m += blockDim.x) {
+    const int k = m % n;
+    double pr = 1, pi = 0;
+    if (ri.s) {
+      const int t = (int)((int64_t)m % (2 * n));
+      sincospi(-(double)t / (double)n, &pi, &pr);
+    }
+    // the equator has no mirror: E = O = its own spectrum
+    double2 a = Dn[k];
+    double2 b = eq ? make_double2(0.0, 0.0) : Ds[k];
+    double er = a.x + b.x, ei = a.y + b.y;
+    double orr = a.x - b.x, oi = a.y - b.y;
+    double *ge = G + (size_t)(2 * m) * plane + (size_t)r * ldg + 2 * c;
+    double *go = ge + plane;
+    ge[0] = w * (er * pr - ei * pi);
+    ge[1] = w * (er * pi + ei * pr);
+    go[0] = w * (orr * pr - oi * pi);
+    go[1] = w * (orr * pi + oi * pr);
+  }
+}
+
+size_t ring_dft_lds_bytes(int n, int lmax)
+{
+  int K = n < lmax + 1 ? n : lmax + 1;
+  return (size_t)(2 * n + 2 * K) * sizeof(double2) + (size_t)n * sizeof(double);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Legendre rows lambda_lm(z_r), l = m..lmax, for the m of one chunk: thread per (m, ring).
+// Start: lambda_mm = (-1)^m 2^{C_m} sin^m(theta), C_m = log2(1/sqrt(4 pi)) + sum_k log2 sqrt((2k+1)/2k);
+// lambda_{m+1,m} = sqrt(2m+3) z lambda_mm;
+// lambda_lm = a_lm (z lambda_{l-1,m} - b_lm lambda_{l-2,m}), a = sqrt((4l^2-1)/(l^2-m^2)),
+// b = sqrt(((l-1)^2-m^2)/(4(l-1)^2-1)).  Values below 2^-300 are carried as v 2^{-600 s}, s >= 1
+// (rescaled upwards when they grow; Holmes & Featherstone 2002, J. Geodesy 76, 279), and enter the
+// table as 0 until s returns to 0.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_legendre(const double2 *__restrict__ coef, const int64_t *__restrict__ off,
+                                                  int m0, int lmax, const double *__restrict__ zs,
+                                                  const double *__restrict__ sts, int R, int RP,
+                                                  double *__restrict__ lam)
+{
+  const int m = m0 + blockIdx.y;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= RP) return;
+  const int L = lmax - m + 1;
+  const int64_t o = off[m];
+  double *out = lam + (size_t)(o - off[m0]) * RP + r;
+  if (r >= R) {
+    for (int i = 0; i < L; i++) out[(size_t)i * RP] = 0.0;
+    return;
+  }
+  const double z = zs[r], st = sts[r];
+  const double ISC = 0x1p-600;
+  double e = coef[o].x + (double)m * log2(st);
+  int s = 0;
+  if (e < -300.0) s = (int)ceil((-300.0 - e) / 600.0);
+  double p = exp2(e + 600.0 * s);
+  if (m & 1) p = -p;
+  out[0] = s == 0 ? p : 0.0;
+  if (L == 1) return;
+  double q = coef[o + 1].x * z * p;
+  out[RP] = s == 0 ? q : 0.0;
+  for (int i = 2; i < L; i++) {
+    const double2 ab = coef[o + i];
+    double t = ab.x * (z * q - ab.y * p);
+    p = q;
+    q = t;
+    if (s > 0 && fabs(q) > 0x1p300) {
+      p *= ISC;
+      q *= ISC;
+      s--;
+    }
+    out[(size_t)i * RP] = s == 0 ? q : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// alm GEMM: for one (m, parity, 64-row tile) and one 64-column tile,
+//   alm[idx(l, m)][col] = sum_r lambda_lm(z_r) G[m][parity][r][col],  l = m + parity + 2 i.
+// 256 threads = 4 waves, each a 32 x 32 block of 2 x 2 v_mfma_f64_16x16x4f64 tiles; K in chunks of 16
+// rings through LDS.  Operand layout of the 16x16x4 FP64 MFMA (64 lanes, lane = i + 16 k for A[i][k],
+// j + 16 k for B[k][j]; D[i][j] in lane j + 16 (i % 4), element i / 4 -- measured on gfx950).
+// ---------------------------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+constexpr int GT = 64, GK = 16;
+
+__global__ __launch_bounds__(256) void k_alm_gemm(const double *__restrict__ lam, const double *__restrict__ G,
+                                                  const int4 *__restrict__ tiles, int ntiles, int ncoltiles,
+                                                  int tiles_per_xcd, const int64_t *__restrict__ off, int m0,
+                                                  int lmax, int RP, int ldg, int ncols, double *__restrict__ alm,
+                                                  int ldo)
+{
+  // XCD-aware order: consecutive logical tiles (which share Legendre rows) stay on one XCD's L2
+  const int b = blockIdx.x;
+  const int logical = (b % 8) * tiles_per_xcd + b / 8;
+  if (logical >= ntiles * ncoltiles) return;
+  const int4 tl = tiles[logical / ncoltiles];
+  const int ct = logical % ncoltiles;
+  const int m = tl.x, par = tl.y, rt = tl.z, nrows = tl.w;
+  __shared__ double As[GT][GK + 1];
+  __shared__ double Bs[GK][GT + 2];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int row0 = rt * GT, col0 = ct * GT;
+  const double *lrow = lam + (size_t)(off[m] - off[m0]) * RP;   // row (l - m) of this m
+  const size_t plane = (size_t)RP * ldg;
+  const double *g = G + (size_t)(2 * m + par) * plane;
+  dbl4 acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0, 0, 0, 0};
+  const int ar = tid >> 2, ak = (tid & 3) * 4;       // A loads: row ar, 4 rings
+  const int bk = tid >> 4, bc = (tid & 15) * 4;      // B loads: ring bk, 4 columns
+  const bool arow_ok = row0 + ar < nrows;
+  const double *ap = lrow + (size_t)(par + 2 * (row0 + ar)) * RP + ak;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  for (int k0 = 0; k0 < RP; k0 += GK) {
+    if (arow_ok) {
+      double2 v0 = *(const double2 *)(ap + k0), v1 = *(const double2 *)(ap + k0 + 2);
+      As[ar][ak] = v0.x; As[ar][ak + 1] = v0.y; As[ar][ak + 2] = v1.x; As[ar][ak + 3] = v1.y;
+    } else {
+      As[ar][ak] = As[ar][ak + 1] = As[ar][ak + 2] = As[ar][ak + 3] = 0.0;
+    }
+    const double *bp = g + (size_t)(k0 + bk) * ldg + col0 + bc;
+    for (int e = 0; e < 4; e++) Bs[bk][bc + e] = (col0 + bc + e < ncols) ? bp[e] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; kk += 4) {
+      double af[2], bf[2];
+      for (int i = 0; i < 2; i++) af[i] = As[wr + 16 * i + (lane & 15)][kk + (lane >> 4)];
+      for (int j = 0; j < 2; j++) bf[j] = Bs[kk + (lane >> 4)][wc + 16 * j + (lane & 15)];
+      for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int64_t base_idx = (int64_t)m * lmax - (int64_t)m * (m - 1) / 2;   // he_indexlm(l, m) - l
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int v = 0; v < 4; v++) {
+        const int row = row0 + wr + 16 * i + (lane >> 4) + 4 * v;
+        const int col = col0 + wc + 16 * j + (lane & 15);
+        if (row < nrows && col < ncols) {
+          const int l = m + par + 2 * row;
+          const int64_t idx = m > 0 ? base_idx + l : l;
+          alm[(size_t)idx * ldo + col] = acc[i][j][v];
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------------------------
+// C_l from alm[idx][2 t + (0 re | 1 im)]: cross (16 x 16 map pairs per workgroup) or autospectra.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cls_cross(const double *__restrict__ alm, int lmax, int nmaps, int ldo,
+                                                   double *__restrict__ cls)
+{
+  const int l = blockIdx.z;
+  const int a0 = blockIdx.x * 16, b0 = blockIdx.y * 16;
+  const int ta = threadIdx.x & 15, tb = threadIdx.x >> 4;
+  __shared__ double2 sa[16][16], sb[16][16];   // [m in chunk][map]
+  double acc = 0;
+  for (int mc = 0; mc <= l; mc += 16) {
+    {
+      const int mm = threadIdx.x >> 4, t = threadIdx.x & 15, m = mc + mm;
+      double2 va = make_double2(0, 0), vb = make_double2(0, 0);
+      if (m <= l) {
+        const int64_t idx = m > 0 ? l + (int64_t)m * lmax - (int64_t)m * (m - 1) / 2 : l;
+        const double *row = alm + (size_t)idx * ldo;
+        const double wgt = m ? 2.0 : 1.0;
+        if (a0 + t < nmaps) va = make_double2(wgt * row[2 * (a0 + t)], wgt * row[2 * (a0 + t) + 1]);
+        if (b0 + t < nmaps) vb = make_double2(row[2 * (b0 + t)], row[2 * (b0 + t) + 1]);
+      }
+      sa[mm][t] = va;
+      sb[mm][t] = vb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int mm = 0; mm < 16; mm++) {
+      double2 x = sa[mm][ta], y = sb[mm][tb];
+      acc = fma(x.x, y.x, fma(x.y, y.y, acc));
+    }
+    __syncthreads();
+  }
+  if (a0 + ta < nmaps && b0 + tb < nmaps)
+    cls[((size_t)(a0 + ta) * nmaps + (b0 + tb)) * (lmax + 1) + l] = acc / (2.0 * l + 1.0);
+}
+
+__global__ __launch_bounds__(256) void k_cls_auto(const double *__restrict__ alm, int lmax, int nmaps, int ldo,
+                                                  double *__restrict__ cls)
+{
+  const int l = blockIdx.y;
+  const int a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= nmaps) return;
+  double acc = 0;
+  for (int m = 0; m <= l; m++) {
+    const int64_t idx = m > 0 ? l + (int64_t)m * lmax - (int64_t)m * (m - 1) / 2 : l;
+    const double *row = alm + (size_t)idx * ldo + 2 * a;
+    const double p = row[0] * row[0] + row[1] * row[1];
+    acc += m ? 2.0 * p : p;
+  }
+  cls[(size_t)a * (lmax + 1) + l] = acc / (2.0 * l + 1.0);
+}
+
+// alm[idx][2 t + c] -> out[t][idx][c]
+__global__ __launch_bounds__(256) void k_alm_unpack(const double *__restrict__ alm, int64_t nalm, int nmaps, int ldo,
+                                                    double *__restrict__ out)
+{
+  const int t = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nalm; i += (int64_t)gridDim.x * 256) {
+    const double *src = alm + (size_t)i * ldo + 2 * t;
+    double *dst = out + ((size_t)t * nalm + i) * 2;
+    dst[0] = src[0];
+    dst[1] = src[1];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------
+hipError_t sht_set_lds_limits(size_t bytes)
+{
+  hipError_t e = hipFuncSetAttribute((const void *)k_ring_dft<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)k_ring_dft<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
+
+hipError_t launch_ring_dft(const void *maps, int fp64, int nmaps, int64_t npix, const ShtRing *rings, int R, int RP,
+                           int lmax, int ldg, double w, size_t lds, double *G, hipStream_t s)
+{
+  dim3 grid(nmaps, R);
+  if (fp64)
+    hipLaunchKernelGGL(k_ring_dft<double>, grid, dim3(512), lds, s, (const double *)maps, npix, rings, R, RP, lmax,
+                       ldg, w, G);
+  else
+    hipLaunchKernelGGL(k_ring_dft<float>, grid, dim3(512), lds, s, (const float *)maps, npix, rings, R, RP, lmax,
+                       ldg, w, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_legendre(const double2 *coef, const int64_t *off, int m0, int m1, int lmax, const double *zs,
+                           const double *sts, int R, int RP, double *lam, hipStream_t s)
+{
+  dim3 grid((RP + 255) / 256, m1 - m0);
+  hipLaunchKernelGGL(k_legendre, grid, dim3(256), 0, s, coef, off, m0, lmax, zs, sts, R, RP, lam);
+  return hipGetLastError();
+}
+
+hipError_t launch_alm_gemm(const double *lam, const double *G, const int4 *tiles, int ntiles, const int64_t *off,
+                           int m0, int lmax, int RP, int ldg, int ncols, double *alm, int ldo, hipStream_t s)
+{
+  const int ncoltiles = (ncols + GT - 1) / GT;
+  const long total = (long)ntiles * ncoltiles;
+  const int per_xcd = (int)((total + 7) / 8);
+  hipLaunchKernelGGL(k_alm_gemm, dim3(8 * per_xcd), dim3(256), 0, s, lam, G, tiles, ntiles, ncoltiles, per_xcd, off,
+                     m0, lmax, RP, ldg, ncols, alm, ldo);
+  return hipGetLastError();
+}
+
+hipError_t launch_cls(const double *alm, int lmax, int nmaps, int ldo, int cross, double *cls, hipStream_t s)
+{
+  if (cross) {
+    dim3 grid((nmaps + 15) / 16, (nmaps + 15) / 16, lmax + 1);
+    hipLaunchKernelGGL(k_cls_cross, grid, dim3(256), 0, s, alm, lmax, nmaps, ldo, cls);
+  } else {
+    dim3 grid((nmaps + 255) / 256, lmax + 1);
+    hipLaunchKernelGGL(k_cls_auto, grid, dim3(256), 0, s, alm, lmax, nmaps, ldo, cls);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_alm_unpack(const double *alm, int64_t nalm, int nmaps, int ldo, double *out, hipStream_t s)
+{
+  int64_t nb = (nalm + 255) / 256;
+  dim3 grid((unsigned)(nb < 4096 ? nb : 4096), nmaps);
+  hipLaunchKernelGGL(k_alm_unpack, grid, dim3(256), 0, s, alm, nalm, nmaps, ldo, out);
+  return hipGetLastError();
+}
+
+}  // namespace crime

@@ -1,0 +1,67 @@
+/*
+ * crime_sht.h -- HEALPix analysis (map2alm) and angular power spectra on one MI355X, in libcrime_gh.so.
+ *
+ * Replaces he_map2alm (src/healpix_extra.c:88-130), which calls libsharp with
+ * sharp_make_triangular_alm_info(lmax, lmax, 1) and sharp_make_weighted_healpix_geom_info(nside, 1,
+ * NULL): every pixel weighs 4 pi / Npix (no quadrature ring weights, no iterations), so
+ *     a_lm = (4 pi / Npix) sum_p f(p) Y*_lm(theta_p, phi_p),     0 <= m <= l <= lmax,
+ * with orthonormal Y_lm carrying the Condon-Shortley phase.  Host alms follow he_indexlm
+ * (src/healpix_extra.c:36-42): a_lm of map t at alms[t][l + m lmax - m (m - 1) / 2], complex as
+ * (re, im) doubles -- the reference's fcomplex of its double-precision build.
+ *
+ * The angular power spectra (no counterpart in the reference's GetHI; the reference's ForGet draws
+ * maps from given C_l, src/gsync_qu.c) are
+ *     C_l^{ab} = ( a_l0^a a_l0^b + 2 sum_{m>0} Re(a_lm^a conj a_lm^b) ) / (2 l + 1).
+ *
+ * All arithmetic is FP64.  A plan owns one HIP stream and its device buffers (the Legendre table of
+ * the plan's Nside and lmax is kept resident when it fits the plan's budget); plans are not shared
+ * across threads.  Status codes are those of crime_gh.h.
+ */
+#ifndef CRIME_SHT_H
+#define CRIME_SHT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "crime_gh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct crime_sht crime_sht_t;
+
+/* Plan for RING maps of the given Nside (1..8192) up to lmax (0..4 nside - 1; 3 nside - 1 is the
+ * usual choice) and up to max_maps maps per call, on HIP device `device`.  The device buffers hold
+ * the ring spectra ((lmax+1) x 2 x 2 nside x max_maps complex doubles), the alms of max_maps maps
+ * and the Legendre table or a chunk of it. */
+int crime_sht_create(int64_t nside, int lmax, int max_maps, int device, crime_sht_t **out);
+void crime_sht_destroy(crime_sht_t *p);
+const char *crime_sht_last_error(const crime_sht_t *p); /* NULL plan: the last failed create */
+
+/* he_map2alm (src/healpix_extra.c:88-130) for nmaps <= max_maps host maps maps[t][Npix] (RING)
+ * into host alms[t][(lmax+1)(lmax+2)/2] (he_indexlm order, interleaved re, im). */
+int crime_sht_map2alm(crime_sht_t *p, int nmaps, const double *maps, double *alms);
+
+/* Same, from maps already in device memory: d_maps[t * Npix + pix], FP64 (maps_fp64 = 1) or FP32
+ * (maps_fp64 = 0), e.g. the resident maps of crime_gh_device_maps. */
+int crime_sht_map2alm_device(crime_sht_t *p, int nmaps, const void *d_maps, int maps_fp64, double *alms);
+
+/* Angular power spectra of device maps.  cross = 1: every pair, cls[(a * nmaps + b) * (lmax + 1) + l];
+ * cross = 0: autospectra only, cls[a * (lmax + 1) + l]. */
+int crime_sht_anafast_device(crime_sht_t *p, int nmaps, const void *d_maps, int maps_fp64, int cross,
+                             double *cls);
+
+/* Same, for nmaps <= max_maps host maps maps[t][Npix] (FP64, RING). */
+int crime_sht_anafast(crime_sht_t *p, int nmaps, const double *maps, int cross, double *cls);
+
+/* Milliseconds of the last call's stages (ring DFTs, Legendre table, alm GEMMs, spectra). */
+typedef struct crime_sht_stats {
+  double ms_rings, ms_legendre, ms_alm, ms_cls, ms_total;
+} crime_sht_stats;
+int crime_sht_get_stats(const crime_sht_t *p, crime_sht_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

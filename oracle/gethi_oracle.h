@@ -29,4 +29,11 @@ void orc_sample_indices(const crime_gh_desc *d, const double *dzrsd, int64_t cel
 int orc_mk_T_maps(const crime_gh_desc *d, const double *mass, const double *dzrsd, double *maps);
 int orc_run(const crime_gh_desc *d, double *maps, double *sigma2, double *mean);
 int orc_num_threads(void);
+
+/* sht_oracle.c: direct-sum HEALPix map2alm and angular power spectra */
+void orc_ring_info(int64_t nside, int64_t ir, int64_t *nphi, int64_t *first, double *z, double *st, int *s);
+void orc_legendre(int lmax, int m, double x, double st, double *out);
+int64_t orc_alm_index(int l, int m, int lmax);
+int orc_map2alm(int64_t nside, int lmax, int nmaps, const double *maps, double *alms);
+void orc_alm2cl(int lmax, int nmaps, const double *alms, double *cls);
 #endif

@@ -54,6 +54,14 @@ def lib():
             getattr(L, f).restype = C.c_double
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
         L.orc_num_threads.restype = C.c_int
+        I64 = C.c_int64
+        L.orc_ring_info.argtypes = [I64, I64, C.POINTER(I64), C.POINTER(I64), _dp, _dp, C.POINTER(C.c_int)]
+        L.orc_legendre.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp]
+        L.orc_alm_index.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_alm_index.restype = I64
+        L.orc_map2alm.argtypes = [I64, C.c_int, C.c_int, _dp, _dp]
+        L.orc_map2alm.restype = C.c_int
+        L.orc_alm2cl.argtypes = [C.c_int, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -174,3 +182,43 @@ def z_of_r(desc, r):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+# ---------------------------------------------------------------------------------------------
+# HEALPix map2alm / C_l (sht_oracle.c)
+# ---------------------------------------------------------------------------------------------
+def ring_info(nside, ir):
+    """(nphi, first pixel, z, sin theta, s) of RING ring ir (1..4 nside-1); phi_j = pi (2j+s)/nphi."""
+    nphi, first, s = C.c_int64(), C.c_int64(), C.c_int()
+    z, st = C.c_double(), C.c_double()
+    lib().orc_ring_info(nside, ir, C.byref(nphi), C.byref(first), C.byref(z), C.byref(st), C.byref(s))
+    return nphi.value, first.value, z.value, st.value, s.value
+
+
+def legendre(lmax, m, x, st):
+    """lambda_lm(x), l = m..lmax (Y_lm = lambda_lm(cos theta) e^{i m phi}, Condon-Shortley phase)."""
+    out = np.empty(lmax - m + 1)
+    lib().orc_legendre(lmax, m, x, st, _p(out))
+    return out
+
+
+def alm_index(l, m, lmax):
+    return lib().orc_alm_index(l, m, lmax)
+
+
+def map2alm(maps, nside, lmax):
+    """maps (nmaps, 12 nside^2) RING -> alms (nmaps, (lmax+1)(lmax+2)/2) complex, m-major."""
+    m = np.ascontiguousarray(np.atleast_2d(maps), dtype=np.float64)
+    nalm = (lmax + 1) * (lmax + 2) // 2
+    out = np.empty((m.shape[0], nalm), np.complex128)
+    rc = lib().orc_map2alm(nside, lmax, m.shape[0], _p(m), out.ctypes.data_as(_dp))
+    assert rc == 0
+    return out
+
+
+def alm2cl(alms, lmax):
+    """cls[a, b, l] for every pair of the alms' maps."""
+    a = np.ascontiguousarray(np.atleast_2d(alms), dtype=np.complex128)
+    out = np.empty((a.shape[0], a.shape[0], lmax + 1))
+    lib().orc_alm2cl(lmax, a.shape[0], a.ctypes.data_as(_dp), _p(out))
+    return out

@@ -19,14 +19,15 @@ def declared(header):
     txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", txt, flags=re.M)
-    return sorted(set(n for n in names if n.startswith(("crime_gh_", "gh_"))))
+    return sorted(set(n for n in names if n.startswith(("crime_gh_", "crime_sht_", "gh_"))))
 
 
-@pytest.mark.parametrize("header,lib", [("crime_gh.h", "gpu"), ("gethi_host.h", "host")])
+@pytest.mark.parametrize("header,lib", [("crime_gh.h", "gpu"), ("crime_sht.h", "gpu"),
+                                        ("gethi_host.h", "host")])
 def test_every_declared_symbol_is_exported(header, lib):
     L = _ffi.gpu_lib() if lib == "gpu" else _ffi.host_lib()
     names = declared(header)
-    assert len(names) > 10
+    assert len(names) >= 7
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 
@@ -90,3 +91,19 @@ def test_cli_errors(tmp_path):
     bad.write_text("n_grid=64\n")
     r = subprocess.run([exe, str(bad)], capture_output=True, text=True)
     assert r.returncode == 1 and "Error reading file" in r.stderr
+
+
+def test_sht_null_and_bad_plans():
+    lib = _ffi.gpu_lib()
+    h = C.c_void_p()
+    assert lib.crime_sht_map2alm(None, 1, None, None) == -1
+    assert lib.crime_sht_anafast_device(None, 1, None, 1, 1, None) == -1
+    lib.crime_sht_destroy(None)
+    # bad geometry is rejected before any device work
+    assert lib.crime_sht_create(0, 10, 1, 0, C.byref(h)) == -1
+    assert h.value is None
+    assert lib.crime_sht_create(16, 64, 1, 0, C.byref(h)) == -1      # lmax > 4 nside - 1
+    assert b"lmax" in lib.crime_sht_last_error(None)
+    assert lib.crime_sht_create(16, 47, 0, 0, C.byref(h)) == -1      # no maps
+    assert lib.crime_sht_create(1024, 3071, 1, 0, C.byref(h)) == -1  # ring spectra beyond the LDS
+    assert b"LDS" in lib.crime_sht_last_error(None)
