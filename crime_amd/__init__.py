@@ -309,10 +309,20 @@ class Sht:
                                                        _dptr(out)))
         return out
 
-    def cls_of(self, gh, cross=True, channels=None):
-        """C_l of a GetHI handle's resident maps (all channels, or the first `channels`)."""
+    def cls_pairs_device(self, dptr, nmaps, fp64, pairs):
+        """C_l of the listed (a, b) map pairs of nmaps device maps: (npairs, lmax+1)."""
+        pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        out = np.empty((pr.shape[0], self.lmax + 1))
+        self._check(self._lib.crime_sht_cls_pairs_device(self._p, nmaps, C.c_void_p(dptr), int(fp64), pr.shape[0],
+                                                          pr.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(out)))
+        return out
+
+    def cls_of(self, gh, cross=True, channels=None, pairs=None):
+        """C_l of a GetHI handle's resident maps: all pairs (cross), autospectra, or the listed pairs."""
         p, _ = gh.device_maps()
         n = gh.n_nu if channels is None else int(channels)
+        if pairs is not None:
+            return self.cls_pairs_device(p, n, gh.precision == FP64, pairs)
         return self.anafast_device(p, n, gh.precision == FP64, cross)
 
     def stats(self):

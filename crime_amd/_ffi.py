@@ -174,10 +174,34 @@ def gpu_lib():
         lib.crime_sht_map2alm_device.argtypes = [H, C.c_int, C.c_void_p, C.c_int, _dp]
         lib.crime_sht_anafast_device.restype = C.c_int
         lib.crime_sht_anafast_device.argtypes = [H, C.c_int, C.c_void_p, C.c_int, C.c_int, _dp]
+        lib.crime_sht_cls_pairs_device.restype = C.c_int
+        lib.crime_sht_cls_pairs_device.argtypes = [H, C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int32), _dp]
         lib.crime_sht_anafast.restype = C.c_int
         lib.crime_sht_anafast.argtypes = [H, C.c_int, _dp, C.c_int, _dp]
         lib.crime_sht_get_stats.restype = C.c_int
         lib.crime_sht_get_stats.argtypes = [H, C.POINTER(CrimeShtStats)]
+        lib.crime_mlp_param_count.restype = C.c_int64
+        lib.crime_mlp_param_count.argtypes = []
+        lib.crime_mlp_create.restype = C.c_int
+        lib.crime_mlp_create.argtypes = [C.c_int, C.c_int, C.POINTER(H)]
+        lib.crime_mlp_destroy.restype = None
+        lib.crime_mlp_destroy.argtypes = [H]
+        lib.crime_mlp_last_error.restype = C.c_char_p
+        lib.crime_mlp_last_error.argtypes = [H]
+        _fp = C.POINTER(C.c_float)
+        lib.crime_mlp_set_params.restype = C.c_int
+        lib.crime_mlp_set_params.argtypes = [H, _fp]
+        lib.crime_mlp_get_params.restype = C.c_int
+        lib.crime_mlp_get_params.argtypes = [H, _fp]
+        lib.crime_mlp_forward.restype = C.c_int
+        lib.crime_mlp_forward.argtypes = [H, C.c_int64, _fp, _fp]
+        lib.crime_mlp_loss_grad.restype = C.c_int
+        lib.crime_mlp_loss_grad.argtypes = [H, C.c_int, _fp, _fp, _dp, _fp]
+        lib.crime_mlp_set_data.restype = C.c_int
+        lib.crime_mlp_set_data.argtypes = [H, C.c_int64, _fp, _fp]
+        lib.crime_mlp_train.restype = C.c_int
+        lib.crime_mlp_train.argtypes = [H, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_float, C.c_float,
+                                        C.c_uint32, _fp]
         if lib.crime_gh_desc_size() != C.sizeof(CrimeGhDesc):
             raise ImportError("crime_gh_desc layout mismatch between libcrime_gh.so and crime_amd._ffi")
         _gpu = lib

@@ -351,6 +351,34 @@ extern "C" int crime_sht_anafast_device(crime_sht_t *p, int nmaps, const void *d
   return 0;
 }
 
+extern "C" int crime_sht_cls_pairs_device(crime_sht_t *p, int nmaps, const void *d_maps, int maps_fp64, int npairs,
+                                          const int32_t *pairs, double *cls)
+{
+  if (!p) return sht_error(nullptr, CRIME_GH_EINVAL, "plan is NULL");
+  if (!pairs || !cls || npairs < 1 || npairs > 65535) return sht_error(p, CRIME_GH_EINVAL, "bad pair list");
+  for (int i = 0; i < 2 * npairs; i++)
+    if (pairs[i] < 0 || pairs[i] >= nmaps) return sht_error(p, CRIME_GH_EINVAL, "pair index %d outside [0, %d)", pairs[i], nmaps);
+  SHT_HIP(p, hipSetDevice(p->device));
+  int rc = analyse(p, nmaps, d_maps, maps_fp64);
+  if (rc) return rc;
+  const size_t n = (size_t)npairs * (p->lmax + 1);
+  const size_t need = n + (size_t)npairs;  // spectra, then the pair list (as int32 pairs in doubles' room)
+  if (need > p->cls_cap) {
+    if (p->d_cls) SHT_HIP(p, hipFree(p->d_cls));
+    p->d_cls = nullptr;
+    p->cls_cap = 0;
+    SHT_HIP(p, hipMalloc((void **)&p->d_cls, need * sizeof(double)));
+    p->cls_cap = need;
+  }
+  int *d_pairs = (int *)(p->d_cls + n);
+  SHT_HIP(p, hipMemcpyAsync(d_pairs, pairs, 2 * (size_t)npairs * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
+  SHT_HIP(p, launch_cls_pairs(p->d_alm, p->lmax, p->ldo, npairs, d_pairs, p->d_cls, p->stream));
+  rc = finish(p, true);
+  if (rc) return rc;
+  SHT_HIP(p, hipMemcpy(cls, p->d_cls, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 extern "C" int crime_sht_anafast(crime_sht_t *p, int nmaps, const double *maps, int cross, double *cls)
 {
   if (!p) return sht_error(nullptr, CRIME_GH_EINVAL, "plan is NULL");

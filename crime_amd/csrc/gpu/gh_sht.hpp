@@ -23,6 +23,8 @@ hipError_t launch_legendre(const double2 *coef, const int64_t *off, int m0, int 
 hipError_t launch_alm_gemm(const double *lam, const double *G, const int4 *tiles, int ntiles, const int64_t *off,
                            int m0, int lmax, int RP, int ldg, int ncols, double *alm, int ldo, hipStream_t s);
 hipError_t launch_cls(const double *alm, int lmax, int nmaps, int ldo, int cross, double *cls, hipStream_t s);
+hipError_t launch_cls_pairs(const double *alm, int lmax, int ldo, int npairs, const int *pairs, double *cls,
+                            hipStream_t s);
 hipError_t launch_alm_unpack(const double *alm, int64_t nalm, int nmaps, int ldo, double *out, hipStream_t s);
 
 }  // namespace crime

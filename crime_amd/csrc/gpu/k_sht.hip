@@ -302,6 +302,23 @@ __global__ __launch_bounds__(256) void k_cls_auto(const double *__restrict__ alm
   cls[(size_t)a * (lmax + 1) + l] = acc / (2.0 * l + 1.0);
 }
 
+// C_l of listed map pairs: one workgroup per (l, pair), 64 threads over m
+__global__ __launch_bounds__(64) void k_cls_pairs(const double *__restrict__ alm, int lmax, int ldo,
+                                                  const int *__restrict__ pairs, double *__restrict__ cls)
+{
+  const int l = blockIdx.x, pr = blockIdx.y;
+  const int a = pairs[2 * pr], b = pairs[2 * pr + 1];
+  double acc = 0;
+  for (int m = threadIdx.x; m <= l; m += 64) {
+    const int64_t idx = m > 0 ? l + (int64_t)m * lmax - (int64_t)m * (m - 1) / 2 : l;
+    const double *row = alm + (size_t)idx * ldo;
+    const double p = row[2 * a] * row[2 * b] + row[2 * a + 1] * row[2 * b + 1];
+    acc += m ? 2.0 * p : p;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (threadIdx.x == 0) cls[(size_t)pr * (lmax + 1) + l] = acc / (2.0 * l + 1.0);
+}
+
 // alm[idx][2 t + c] -> out[t][idx][c]
 __global__ __launch_bounds__(256) void k_alm_unpack(const double *__restrict__ alm, int64_t nalm, int nmaps, int ldo,
                                                     double *__restrict__ out)
@@ -368,6 +385,13 @@ hipError_t launch_cls(const double *alm, int lmax, int nmaps, int ldo, int cross
     dim3 grid((nmaps + 255) / 256, lmax + 1);
     hipLaunchKernelGGL(k_cls_auto, grid, dim3(256), 0, s, alm, lmax, nmaps, ldo, cls);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_cls_pairs(const double *alm, int lmax, int ldo, int npairs, const int *pairs, double *cls,
+                            hipStream_t s)
+{
+  hipLaunchKernelGGL(k_cls_pairs, dim3(lmax + 1, npairs), dim3(64), 0, s, alm, lmax, ldo, pairs, cls);
   return hipGetLastError();
 }
 

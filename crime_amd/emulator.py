@@ -1,0 +1,247 @@
+"""Cross-frequency C_l emulator trained on ensembles of this pipeline.
+
+Inputs (Omega_m, sigma_8, n_s, h, b, nu, nu', log10 l) -> output log C_l(nu, nu') of binned angular
+spectra of the HI temperature maps; network and training run on the GPU through include/crime_mlp.h
+(BF16 MFMA, FP32 accumulation, Adam).  The plain pipeline (GetHI maps + map2alm of include/crime_sht.h)
+produces the training targets and stays the method the emulator is judged against.
+
+A "mock" here is one GetHI run: Philox Gaussian field of the cosmology's BBKS P(k), lognormal HI with a
+constant bias b, RSD, pixelisation into the frequency channels; its C_l are measured per channel pair and
+averaged in logarithmic l bins.
+"""
+import ctypes as C
+import math
+import tempfile
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import CrimeError, GetHI, Params, Sht, _ffi
+from . import configs as cf
+
+NIN = 8
+# parameter ranges of the cosmologies (Latin hypercube): Omega_m, sigma_8, n_s, h, b
+RANGES = np.array([[0.25, 0.35], [0.70, 0.90], [0.92, 1.00], [0.60, 0.80], [0.80, 1.20]])
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The 8 -> 256 x 5 (SiLU) -> 1 network of libcrime_gh.so (include/crime_mlp.h)."""
+
+    def __init__(self, max_batch=8192, device=0):
+        self._lib = _ffi.gpu_lib()
+        self.nparam = int(self._lib.crime_mlp_param_count())
+        self.max_batch = int(max_batch)
+        h = C.c_void_p()
+        rc = self._lib.crime_mlp_create(self.max_batch, device, C.byref(h))
+        if rc:
+            raise CrimeError(rc, self._lib.crime_mlp_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc:
+            raise CrimeError(rc, self._lib.crime_mlp_last_error(self._h).decode())
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self._check(self._lib.crime_mlp_set_params(self._h, _fp(p)))
+
+    def params(self):
+        out = np.empty(self.nparam, np.float32)
+        self._check(self._lib.crime_mlp_get_params(self._h, _fp(out)))
+        return out
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NIN)
+        y = np.empty(x.shape[0], np.float32)
+        self._check(self._lib.crime_mlp_forward(self._h, x.shape[0], _fp(x), _fp(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NIN)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.nparam, np.float32)
+        loss = C.c_double()
+        self._check(self._lib.crime_mlp_loss_grad(self._h, x.shape[0], _fp(x), _fp(t), C.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def set_data(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NIN)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        self._check(self._lib.crime_mlp_set_data(self._h, x.shape[0], _fp(x), _fp(t)))
+
+    def train(self, steps, batch, lr0, total_steps, seed=0, beta1=0.9, beta2=0.999, eps=1e-8):
+        loss = np.empty(steps, np.float32)
+        self._check(self._lib.crime_mlp_train(self._h, steps, batch, lr0, total_steps, beta1, beta2, eps, seed,
+                                              _fp(loss)))
+        return loss
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.crime_mlp_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+def latin_hypercube(n, seed, ranges=RANGES):
+    """n points, one per stratum in every dimension (McKay, Beckman & Conover 1979)."""
+    rng = np.random.default_rng(seed)
+    d = ranges.shape[0]
+    u = (np.argsort(rng.random((d, n)), axis=1).T + rng.random((n, d))) / n
+    return ranges[:, 0] + u * (ranges[:, 1] - ranges[:, 0])
+
+
+class MockSpec:
+    """Grid, sky and spectrum binning of the emulated mocks."""
+
+    def __init__(self, n_grid=256, n_side=256, n_nu=64, nu_lo=700.0, nu_hi=800.0, channels=(0, 9, 18, 27, 36, 45, 54, 62),
+                 l_edges=None):
+        self.n_grid, self.n_side, self.n_nu = n_grid, n_side, n_nu
+        self.nu_lo, self.nu_hi = nu_lo, nu_hi
+        self.lmax = 3 * n_side - 1
+        # auto spectra of the listed channels and their cross spectra with the next channel
+        self.pairs = [(c, c) for c in channels] + [(c, c + 1) for c in channels]
+        lo = 8
+        hi = min(n_side, self.lmax)
+        self.l_edges = np.unique(np.round(np.geomspace(lo, hi, 17)).astype(int)) if l_edges is None else np.asarray(l_edges)
+        nuc = 0.5 * (np.linspace(nu_lo, nu_hi, n_nu + 1)[1:] + np.linspace(nu_lo, nu_hi, n_nu + 1)[:-1])
+        self.nu_centres = nuc
+
+    @property
+    def nbins(self):
+        return len(self.l_edges) - 1
+
+    def bin(self, cl):
+        """Mean of C_l over [edge_i, edge_{i+1}) weighted by 2l+1 (per pair)."""
+        out = np.empty((cl.shape[0], self.nbins))
+        ls = np.arange(cl.shape[1])
+        for i in range(self.nbins):
+            sel = (ls >= self.l_edges[i]) & (ls < self.l_edges[i + 1])
+            w = 2 * ls[sel] + 1.0
+            out[:, i] = (cl[:, sel] * w).sum(axis=1) / w.sum()
+        return out
+
+    def l_centres(self):
+        e = self.l_edges
+        return np.sqrt(e[:-1] * (e[1:] - 1.0).clip(1))
+
+
+def _params_for(cosmo, spec, tmp, name):
+    om, s8, ns, h, b = cosmo
+    ini = cf.make(tmp, name, n_grid=spec.n_grid, n_side=spec.n_side, n_nu=spec.n_nu, nu_lo=spec.nu_lo,
+                  nu_hi=spec.nu_hi, l_box=-1.0, seed=1, omega_m=om, sigma_8=s8, n_s=ns, h=h, precision="single",
+                  bias_model="const", bias_b0=b)
+    return Params(ini)
+
+
+def simulate(cosmos, nseeds, spec, seed0, log=None):
+    """Binned C_l of every (cosmology, realisation): array (ncosmo, nseeds, npairs, nbins), and the seconds
+    per mock (GetHI set-up on the device, run and spectra; host parameter set-up excluded).  Realisation j
+    of cosmology i uses the Philox seed seed0 + i nseeds + j."""
+    ncos = len(cosmos)
+    out = np.empty((ncos, nseeds, len(spec.pairs), spec.nbins))
+    t_mock = 0.0
+    with tempfile.TemporaryDirectory() as tmp, ThreadPoolExecutor(16) as ex:
+        pars = list(ex.map(lambda i: _params_for(cosmos[i], spec, tmp, f"c{i:04d}"), range(ncos)))
+        with Sht(spec.n_side, spec.lmax, max_maps=spec.n_nu) as sh:
+            for i, par in enumerate(pars):
+                for j in range(nseeds):
+                    t0 = time.perf_counter()
+                    with GetHI(par, seed=int(seed0 + i * nseeds + j)) as gh:
+                        gh.run()
+                        cl = sh.cls_of(gh, pairs=spec.pairs)
+                    t_mock += time.perf_counter() - t0
+                    out[i, j] = spec.bin(cl)
+                par.close()
+                if log and (i + 1) % max(1, ncos // 8) == 0:
+                    log(f"simulated {i + 1}/{ncos} cosmologies")
+    return out, t_mock / (ncos * nseeds)
+
+
+def features(cosmos, spec):
+    """Network inputs of every (cosmology, pair, l bin), raw units."""
+    ncos = len(cosmos)
+    npair, nb = len(spec.pairs), spec.nbins
+    x = np.empty((ncos, npair, nb, NIN))
+    lc = np.log10(spec.l_centres())
+    for p, (a, b) in enumerate(spec.pairs):
+        x[:, p, :, 5] = spec.nu_centres[a]
+        x[:, p, :, 6] = spec.nu_centres[b]
+    x[:, :, :, 7] = lc[None, None, :]
+    x[:, :, :, :5] = np.asarray(cosmos)[:, None, None, :]
+    return x.reshape(-1, NIN)
+
+
+class Emulator:
+    """Standardised inputs/outputs around a trained Mlp."""
+
+    def __init__(self, mlp, x_mean, x_std, y_mean, y_std):
+        self.mlp, self.x_mean, self.x_std, self.y_mean, self.y_std = mlp, x_mean, x_std, y_mean, y_std
+
+    def predict_log(self, x):
+        return self.mlp.forward((x - self.x_mean) / self.x_std) * self.y_std + self.y_mean
+
+
+def train_emulator(cosmos, cls, spec, steps=4000, batch=4096, lr0=2e-3, seed=0, log=None):
+    """Fit log of the seed-averaged binned C_l.  Returns (Emulator, per-step losses)."""
+    x = features(cosmos, spec)
+    y = np.log(cls.mean(axis=1)).reshape(-1)
+    xm, xs = x.mean(axis=0), x.std(axis=0)
+    xs[xs == 0] = 1.0
+    ym, ys = y.mean(), y.std()
+    mlp = Mlp(max_batch=max(batch, 64))
+    mlp.set_params(init_params(seed))
+    mlp.set_data((x - xm) / xs, (y - ym) / ys)
+    losses = mlp.train(steps, batch, lr0, steps, seed=seed)
+    if log:
+        log(f"trained {steps} steps: loss {losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}")
+    return Emulator(mlp, xm, xs, ym, ys), losses
+
+
+def init_params(seed):
+    """He-uniform weights and zero biases, in the parameter order of include/crime_mlp.h."""
+    rng = np.random.default_rng(seed)
+    wd, nin = 256, NIN
+    parts = [rng.uniform(-1, 1, (nin, wd)) * np.sqrt(6.0 / nin), np.zeros(wd)]
+    for _ in range(4):
+        parts += [rng.uniform(-1, 1, (wd, wd)) * np.sqrt(6.0 / wd), np.zeros(wd)]
+    parts += [rng.uniform(-1, 1, wd) * np.sqrt(3.0 / wd), np.zeros(1)]
+    return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+
+
+def evaluate(emu, cosmos_train, cls_train, cosmos_test, cls_test, spec):
+    """err/sigma of the emulator and of the nearest-training-cosmology lookup on held-out cosmologies.
+
+    truth = mean of the held-out mocks; sigma = scatter of one mock's binned C_l across seeds (cosmic
+    variance of a realisation).  The truth itself carries sigma / sqrt(nseeds) of noise; the debiased
+    RMS subtracts it in quadrature."""
+    nseed = cls_test.shape[1]
+    truth = cls_test.mean(axis=1)
+    sigma = cls_test.std(axis=1, ddof=1)
+    pred = np.exp(emu.predict_log(features(cosmos_test, spec))).reshape(truth.shape)
+    z = (pred - truth) / sigma
+    # no-emulator method: the C_l of the nearest training cosmology (standardised parameter distance)
+    span = RANGES[:, 1] - RANGES[:, 0]
+    d = (((np.asarray(cosmos_test)[:, None, :] - np.asarray(cosmos_train)[None, :, :]) / span) ** 2).sum(axis=2)
+    near = cls_train.mean(axis=1)[d.argmin(axis=1)]
+    zn = (near - truth) / sigma
+
+    def summary(zz):
+        rms = float(np.sqrt(np.mean(zz ** 2)))
+        return {"rms": rms, "rms_debiased": float(np.sqrt(max(0.0, rms ** 2 - 1.0 / nseed))),
+                "median_abs": float(np.median(np.abs(zz))), "p95_abs": float(np.percentile(np.abs(zz), 95))}
+
+    return {"emulator": summary(z), "nearest_cosmology": summary(zn), "truth_noise_sigma": 1.0 / math.sqrt(nseed)}

@@ -52,7 +52,12 @@ int crime_sht_map2alm_device(crime_sht_t *p, int nmaps, const void *d_maps, int 
 int crime_sht_anafast_device(crime_sht_t *p, int nmaps, const void *d_maps, int maps_fp64, int cross,
                              double *cls);
 
-/* Same, for nmaps <= max_maps host maps maps[t][Npix] (FP64, RING). */
+/* C_l^{ab} for a list of map pairs only: pairs[2 i], pairs[2 i + 1] index the nmaps device maps;
+ * cls[i * (lmax + 1) + l]. */
+int crime_sht_cls_pairs_device(crime_sht_t *p, int nmaps, const void *d_maps, int maps_fp64, int npairs,
+                               const int32_t *pairs, double *cls);
+
+/* Same as crime_sht_anafast_device, for nmaps <= max_maps host maps maps[t][Npix] (FP64, RING). */
 int crime_sht_anafast(crime_sht_t *p, int nmaps, const double *maps, int cross, double *cls);
 
 /* Milliseconds of the last call's stages (ring DFTs, Legendre table, alm GEMMs, spectra). */

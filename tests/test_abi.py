@@ -19,10 +19,10 @@ def declared(header):
     txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", txt, flags=re.M)
-    return sorted(set(n for n in names if n.startswith(("crime_gh_", "crime_sht_", "gh_"))))
+    return sorted(set(n for n in names if n.startswith(("crime_gh_", "crime_sht_", "crime_mlp_", "gh_"))))
 
 
-@pytest.mark.parametrize("header,lib", [("crime_gh.h", "gpu"), ("crime_sht.h", "gpu"),
+@pytest.mark.parametrize("header,lib", [("crime_gh.h", "gpu"), ("crime_sht.h", "gpu"), ("crime_mlp.h", "gpu"),
                                         ("gethi_host.h", "host")])
 def test_every_declared_symbol_is_exported(header, lib):
     L = _ffi.gpu_lib() if lib == "gpu" else _ffi.host_lib()
@@ -107,3 +107,19 @@ def test_sht_null_and_bad_plans():
     assert lib.crime_sht_create(16, 47, 0, 0, C.byref(h)) == -1      # no maps
     assert lib.crime_sht_create(1024, 3071, 1, 0, C.byref(h)) == -1  # ring spectra beyond the LDS
     assert b"LDS" in lib.crime_sht_last_error(None)
+
+
+def test_mlp_abi_without_device():
+    lib = _ffi.gpu_lib()
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mlp_oracle
+
+    assert lib.crime_mlp_param_count() == mlp_oracle.layout()[1] == 265729
+    h = C.c_void_p()
+    assert lib.crime_mlp_create(100, 0, C.byref(h)) == -1        # not a multiple of 64
+    assert h.value is None
+    assert b"max_batch" in lib.crime_mlp_last_error(None)
+    assert lib.crime_mlp_forward(None, 1, None, None) == -1
+    assert lib.crime_mlp_train(None, 1, 64, 1e-3, 10, 0.9, 0.999, 1e-8, 0, None) == -1
+    lib.crime_mlp_destroy(None)
