@@ -196,9 +196,9 @@ def gpu_lib():
         lib.crime_mlp_forward.restype = C.c_int
         lib.crime_mlp_forward.argtypes = [H, C.c_int64, _fp, _fp]
         lib.crime_mlp_loss_grad.restype = C.c_int
-        lib.crime_mlp_loss_grad.argtypes = [H, C.c_int, _fp, _fp, _dp, _fp]
+        lib.crime_mlp_loss_grad.argtypes = [H, C.c_int, _fp, _fp, _fp, _dp, _fp]
         lib.crime_mlp_set_data.restype = C.c_int
-        lib.crime_mlp_set_data.argtypes = [H, C.c_int64, _fp, _fp]
+        lib.crime_mlp_set_data.argtypes = [H, C.c_int64, _fp, _fp, _fp]
         lib.crime_mlp_train.restype = C.c_int
         lib.crime_mlp_train.argtypes = [H, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_float, C.c_float,
                                         C.c_uint32, _fp]

@@ -24,7 +24,8 @@ constexpr int64_t OFF_W0 = 0, OFF_MID = OFF_W0 + (int64_t)NIN * WD + WD;
 constexpr int64_t MID_STRIDE = (int64_t)WD * WD + WD; // W_k then b_k
 constexpr int64_t OFF_W5 = OFF_MID + NMID * MID_STRIDE;
 constexpr int64_t NPARAM = OFF_W5 + WD + 1;
-constexpr int KSPLIT = 512;                           // batch rows per split of a weight gradient
+constexpr int KSPLIT = 256;                           // batch rows per split of a weight-gradient GEMM
+constexpr int RROWS = 32;                             // batch rows per partial of the column reductions
 constexpr uint32_t MLP_KEY1 = 0x4D4C5031u;            // second Philox key word of the minibatch draws
 
 typedef short s4 __attribute__((ext_vector_type(4)));
@@ -161,9 +162,10 @@ __global__ __launch_bounds__(256) void k_in_fwd(const float *__restrict__ X, con
   H1[(size_t)r * WD + c] = f2bf(z * sigm(z));
 }
 
-// one wave per row: y = h5 . w5 + b5; when training also dy = 2 (y - t) / n, the loss and dz5
+// one wave per row: y = h5 . w5 + b5; when training also dy = 2 w (y - t) / n, the loss and dz5
 __global__ __launch_bounds__(256) void k_out(const uint16_t *__restrict__ H5, const float *__restrict__ Z5,
-                                             const float *__restrict__ p, const float *__restrict__ T, int n,
+                                             const float *__restrict__ p, const float *__restrict__ T,
+                                             const float *__restrict__ Wt, int n,
                                              int train, float *__restrict__ Y, float *__restrict__ dy,
                                              uint16_t *__restrict__ dZ5, const DevState *__restrict__ st,
                                              float *__restrict__ loss_hist)
@@ -180,12 +182,12 @@ __global__ __launch_bounds__(256) void k_out(const uint16_t *__restrict__ H5, co
     if (lane == 0) Y[r] = y;
     return;
   }
-  const float e = y - T[r];
-  const float d = 2.0f * e / (float)n;
+  const float e = y - T[r], wt = Wt[r];
+  const float d = 2.0f * wt * e / (float)n;
   if (lane == 0) {
     Y[r] = y;
     dy[r] = d;
-    atomicAdd(loss_hist + st->local, e * e / (float)n);
+    atomicAdd(loss_hist + st->local, wt * e * e / (float)n);
   }
   for (int c = lane; c < WD; c += 64) {
     const float z = Z5[(size_t)r * WD + c], sg = sigm(z);
@@ -193,12 +195,12 @@ __global__ __launch_bounds__(256) void k_out(const uint16_t *__restrict__ H5, co
   }
 }
 
-// partials of dw5 = sum_r h5 dy and db5 = sum_r dy over rows [z KSPLIT, (z+1) KSPLIT)
+// partials of dw5 = sum_r h5 dy and db5 = sum_r dy over rows [z RROWS, (z+1) RROWS)
 __global__ __launch_bounds__(256) void k_out_wgrad(const uint16_t *__restrict__ H5, const float *__restrict__ dy, int n,
                                                    float *__restrict__ part)
 {
   const int c = threadIdx.x, z = blockIdx.x;
-  const int r0 = z * KSPLIT, r1 = n < r0 + KSPLIT ? n : r0 + KSPLIT;
+  const int r0 = z * RROWS, r1 = n < r0 + RROWS ? n : r0 + RROWS;
   float a = 0.f, b = 0.f;
   for (int r = r0; r < r1; r++) {
     const float d = dy[r];
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(256) void k_out_wgrad(const uint16_t *__restrict__ 
 __global__ __launch_bounds__(256) void k_colsum(const uint16_t *__restrict__ dZ, int n, float *__restrict__ part)
 {
   const int c = threadIdx.x, z = blockIdx.x;
-  const int r0 = z * KSPLIT, r1 = n < r0 + KSPLIT ? n : r0 + KSPLIT;
+  const int r0 = z * RROWS, r1 = n < r0 + RROWS ? n : r0 + RROWS;
   float a = 0.f;
   for (int r = r0; r < r1; r++) a += bf2f(dZ[(size_t)r * WD + c]);
   part[(size_t)z * WD + c] = a;
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(256) void k_in_wgrad(const float *__restrict__ X, c
                                                   float *__restrict__ part)
 {
   const int c = threadIdx.x, z = blockIdx.x;
-  const int r0 = z * KSPLIT, r1 = n < r0 + KSPLIT ? n : r0 + KSPLIT;
+  const int r0 = z * RROWS, r1 = n < r0 + RROWS ? n : r0 + RROWS;
   float a[NIN + 1] = {};
   for (int r = r0; r < r1; r++) {
     const float g = bf2f(dZ1[(size_t)r * WD + c]);
@@ -235,9 +237,9 @@ __global__ __launch_bounds__(256) void k_in_wgrad(const float *__restrict__ X, c
   for (int i = 0; i <= NIN; i++) part[((size_t)z * (NIN + 1) + i) * WD + c] = a[i];
 }
 
-__global__ __launch_bounds__(256) void k_gather(const float *__restrict__ Xall, const float *__restrict__ Tall, int n,
-                                                const DevState *__restrict__ st, float *__restrict__ X,
-                                                float *__restrict__ T)
+__global__ __launch_bounds__(256) void k_gather(const float *__restrict__ Xall, const float *__restrict__ Tall,
+                                                const float *__restrict__ Wall, int n, const DevState *__restrict__ st,
+                                                float *__restrict__ X, float *__restrict__ T, float *__restrict__ Wt)
 {
   const int b = blockIdx.x * 256 + threadIdx.x;
   if (b >= n) return;
@@ -247,6 +249,7 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ Xall, 
   const int64_t i = (int64_t)(u % (uint64_t)st->ndata);
   for (int k = 0; k < NIN; k++) X[(size_t)b * NIN + k] = Xall[(size_t)i * NIN + k];
   T[b] = Tall[i];
+  Wt[b] = Wall[i];
 }
 
 // Adam (Kingma & Ba 2015) on one contiguous parameter segment; g = sum of nsplit partials
@@ -305,14 +308,14 @@ struct crime_mlp {
   hipStream_t stream = nullptr;
   float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
   uint16_t *wbf = nullptr;                  // [NMID][WD][WD]
-  float *X = nullptr, *T = nullptr, *Y = nullptr, *dy = nullptr;
+  float *X = nullptr, *T = nullptr, *Wt = nullptr, *Y = nullptr, *dy = nullptr;
   float *Z = nullptr;                       // [NH][max_batch][WD]
   uint16_t *H = nullptr, *dZ = nullptr;     // [NH][max_batch][WD]
   float *part_in = nullptr, *part_out = nullptr, *part_w = nullptr, *part_b = nullptr;
   DevState *st = nullptr;
   float *loss = nullptr;
   int loss_cap = 0;
-  float *Xall = nullptr, *Tall = nullptr;
+  float *Xall = nullptr, *Tall = nullptr, *Wall = nullptr;
   int64_t ndata = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -360,8 +363,8 @@ extern "C" void crime_mlp_destroy(crime_mlp_t *h)
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graph(h);
-  void *bufs[] = {h->p, h->m, h->v, h->grad, h->wbf, h->X, h->T, h->Y, h->dy, h->Z, h->H, h->dZ,
-                  h->part_in, h->part_out, h->part_w, h->part_b, h->st, h->loss, h->Xall, h->Tall};
+  void *bufs[] = {h->p, h->m, h->v, h->grad, h->wbf, h->X, h->T, h->Wt, h->Y, h->dy, h->Z, h->H, h->dZ,
+                  h->part_in, h->part_out, h->part_w, h->part_b, h->st, h->loss, h->Xall, h->Tall, h->Wall};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -369,6 +372,7 @@ extern "C" void crime_mlp_destroy(crime_mlp_t *h)
 }
 
 static int nsplit_of(int n) { return (n + KSPLIT - 1) / KSPLIT; }
+static int nred_of(int n) { return (n + RROWS - 1) / RROWS; }
 
 static int create_impl(crime_mlp *h, int max_batch, int device)
 {
@@ -382,7 +386,7 @@ static int create_impl(crime_mlp *h, int max_batch, int device)
   MLP_HIP(h, hipSetDevice(device));
   MLP_HIP(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   const size_t B = (size_t)max_batch, act = (size_t)NH * B * WD;
-  const int ns = nsplit_of(max_batch);
+  const int ns = nsplit_of(max_batch), nr = nred_of(max_batch);
   MLP_HIP(h, hipMalloc((void **)&h->p, NPARAM * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->m, NPARAM * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->v, NPARAM * sizeof(float)));
@@ -390,15 +394,16 @@ static int create_impl(crime_mlp *h, int max_batch, int device)
   MLP_HIP(h, hipMalloc((void **)&h->wbf, (size_t)NMID * WD * WD * sizeof(uint16_t)));
   MLP_HIP(h, hipMalloc((void **)&h->X, B * NIN * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->T, B * sizeof(float)));
+  MLP_HIP(h, hipMalloc((void **)&h->Wt, B * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->Y, B * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->dy, B * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->Z, act * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->H, act * sizeof(uint16_t)));
   MLP_HIP(h, hipMalloc((void **)&h->dZ, act * sizeof(uint16_t)));
-  MLP_HIP(h, hipMalloc((void **)&h->part_in, (size_t)ns * (NIN + 1) * WD * sizeof(float)));
-  MLP_HIP(h, hipMalloc((void **)&h->part_out, (size_t)ns * (WD + 1) * sizeof(float)));
+  MLP_HIP(h, hipMalloc((void **)&h->part_in, (size_t)nr * (NIN + 1) * WD * sizeof(float)));
+  MLP_HIP(h, hipMalloc((void **)&h->part_out, (size_t)nr * (WD + 1) * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->part_w, (size_t)NMID * ns * WD * WD * sizeof(float)));
-  MLP_HIP(h, hipMalloc((void **)&h->part_b, (size_t)NMID * ns * WD * sizeof(float)));
+  MLP_HIP(h, hipMalloc((void **)&h->part_b, (size_t)NMID * nr * WD * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->st, sizeof(DevState)));
   DevState s{};
   MLP_HIP(h, hipMemcpy(h->st, &s, sizeof(s), hipMemcpyHostToDevice));
@@ -466,7 +471,7 @@ static void enqueue_forward(crime_mlp *h, int n, int train)
                        h->wbf + (size_t)k * WD * WD, WD, n, WD, WD, e);
   }
   hipLaunchKernelGGL(k_out, dim3((n + 3) / 4), dim3(256), 0, h->stream, h->H + (size_t)(NH - 1) * B * WD,
-                     h->Z + (size_t)(NH - 1) * B * WD, h->p, h->T, n, train, h->Y, h->dy,
+                     h->Z + (size_t)(NH - 1) * B * WD, h->p, h->T, h->Wt, n, train, h->Y, h->dy,
                      h->dZ + (size_t)(NH - 1) * B * WD, h->st, h->loss);
 }
 
@@ -474,8 +479,8 @@ static void enqueue_forward(crime_mlp *h, int n, int train)
 static void enqueue_backward(crime_mlp *h, int n)
 {
   const size_t B = (size_t)h->max_batch;
-  const int ns = nsplit_of(n);
-  hipLaunchKernelGGL(k_out_wgrad, dim3(ns), dim3(256), 0, h->stream, h->H + (size_t)(NH - 1) * B * WD, h->dy, n,
+  const int ns = nsplit_of(n), nr = nred_of(n);
+  hipLaunchKernelGGL(k_out_wgrad, dim3(nr), dim3(256), 0, h->stream, h->H + (size_t)(NH - 1) * B * WD, h->dy, n,
                      h->part_out);
   for (int k = NMID - 1; k >= 0; k--) {
     const uint16_t *Hin = h->H + (size_t)k * B * WD;         // input of square layer k
@@ -483,31 +488,31 @@ static void enqueue_backward(crime_mlp *h, int n)
     EpiPart ep{h->part_w + (size_t)k * nsplit_of(h->max_batch) * WD * WD};
     hipLaunchKernelGGL((k_gemm<true, false, EpiPart>), dim3(WD / TM, WD / TN, ns), dim3(256), 0, h->stream, Hin, WD,
                        dZo, WD, WD, n, KSPLIT, ep);
-    hipLaunchKernelGGL(k_colsum, dim3(ns), dim3(256), 0, h->stream, dZo, n,
-                       h->part_b + (size_t)k * nsplit_of(h->max_batch) * WD);
+    hipLaunchKernelGGL(k_colsum, dim3(nr), dim3(256), 0, h->stream, dZo, n,
+                       h->part_b + (size_t)k * nred_of(h->max_batch) * WD);
     EpiBwd eb{h->Z + (size_t)k * B * WD, h->dZ + (size_t)k * B * WD};
     hipLaunchKernelGGL((k_gemm<false, true, EpiBwd>), dim3((n + TM - 1) / TM, WD / TN, 1), dim3(256), 0, h->stream,
                        dZo, WD, h->wbf + (size_t)k * WD * WD, WD, n, WD, WD, eb);
   }
-  hipLaunchKernelGGL(k_in_wgrad, dim3(ns), dim3(256), 0, h->stream, h->X, h->dZ, n, h->part_in);
+  hipLaunchKernelGGL(k_in_wgrad, dim3(nr), dim3(256), 0, h->stream, h->X, h->dZ, n, h->part_in);
 }
 
-// segments of the parameter vector and their partials
+// segments of the parameter vector and their partials (GEMM splits or row-chunk reductions)
 struct Seg {
   int64_t off, count;
   const float *part;
-  bool bf;
-  int k;
+  bool gemm;
+  int k;   // square layer whose BF16 copy follows the update, or -1
 };
 
 static std::vector<Seg> segments(crime_mlp *h)
 {
-  const size_t nsm = (size_t)nsplit_of(h->max_batch);
+  const size_t nsm = (size_t)nsplit_of(h->max_batch), nrm = (size_t)nred_of(h->max_batch);
   std::vector<Seg> s;
   s.push_back({OFF_W0, (int64_t)(NIN + 1) * WD, h->part_in, false, -1});
   for (int k = 0; k < NMID; k++) {
     s.push_back({OFF_MID + k * MID_STRIDE, (int64_t)WD * WD, h->part_w + k * nsm * WD * WD, true, k});
-    s.push_back({OFF_MID + k * MID_STRIDE + (int64_t)WD * WD, WD, h->part_b + k * nsm * WD, false, -1});
+    s.push_back({OFF_MID + k * MID_STRIDE + (int64_t)WD * WD, WD, h->part_b + k * nrm * WD, false, -1});
   }
   s.push_back({OFF_W5, WD + 1, h->part_out, false, -1});
   return s;
@@ -515,10 +520,10 @@ static std::vector<Seg> segments(crime_mlp *h)
 
 static void enqueue_adam(crime_mlp *h, int n)
 {
-  const int ns = nsplit_of(n);
   for (const Seg &g : segments(h))
     hipLaunchKernelGGL(k_adam, dim3(nblk(g.count)), dim3(256), 0, h->stream, h->p + g.off, h->m + g.off, h->v + g.off,
-                       g.bf ? h->wbf + (size_t)g.k * WD * WD : (uint16_t *)nullptr, g.part, ns, g.count, h->st);
+                       g.k >= 0 ? h->wbf + (size_t)g.k * WD * WD : (uint16_t *)nullptr, g.part,
+                       g.gemm ? nsplit_of(n) : nred_of(n), g.count, h->st);
   hipLaunchKernelGGL(k_step_done, dim3(1), dim3(64), 0, h->stream, h->st);
 }
 
@@ -552,7 +557,8 @@ extern "C" int crime_mlp_forward(crime_mlp_t *h, int64_t n, const float *x, floa
   return 0;
 }
 
-extern "C" int crime_mlp_loss_grad(crime_mlp_t *h, int n, const float *x, const float *t, double *loss, float *grad)
+extern "C" int crime_mlp_loss_grad(crime_mlp_t *h, int n, const float *x, const float *t, const float *w,
+                                   double *loss, float *grad)
 {
   if (!h) return mlp_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
   if (!x || !t || !loss || !grad) return mlp_error(h, CRIME_GH_EINVAL, "NULL argument");
@@ -570,11 +576,17 @@ extern "C" int crime_mlp_loss_grad(crime_mlp_t *h, int n, const float *x, const 
   MLP_HIP(h, hipMemcpy(h->loss, &zero, sizeof(float), hipMemcpyHostToDevice));
   MLP_HIP(h, hipMemcpy(h->X, x, (size_t)n * NIN * sizeof(float), hipMemcpyHostToDevice));
   MLP_HIP(h, hipMemcpy(h->T, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  if (w) {
+    MLP_HIP(h, hipMemcpy(h->Wt, w, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> ones((size_t)n, 1.0f);
+    MLP_HIP(h, hipMemcpy(h->Wt, ones.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  }
   enqueue_forward(h, n, 1);
   enqueue_backward(h, n);
-  const int ns = nsplit_of(n);
   for (const Seg &g : segments(h))
-    hipLaunchKernelGGL(k_grad_out, dim3(nblk(g.count)), dim3(256), 0, h->stream, g.part, ns, g.count, h->grad + g.off);
+    hipLaunchKernelGGL(k_grad_out, dim3(nblk(g.count)), dim3(256), 0, h->stream, g.part,
+                       g.gemm ? nsplit_of(n) : nred_of(n), g.count, h->grad + g.off);
   MLP_HIP(h, hipGetLastError());
   MLP_HIP(h, hipStreamSynchronize(h->stream));
   float l = 0;
@@ -586,7 +598,7 @@ extern "C" int crime_mlp_loss_grad(crime_mlp_t *h, int n, const float *x, const 
   return 0;
 }
 
-extern "C" int crime_mlp_set_data(crime_mlp_t *h, int64_t n, const float *x, const float *t)
+extern "C" int crime_mlp_set_data(crime_mlp_t *h, int64_t n, const float *x, const float *t, const float *w)
 {
   if (!h) return mlp_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
   if (n < 1 || n > ((int64_t)1 << 40) || !x || !t) return mlp_error(h, CRIME_GH_EINVAL, "bad training set");
@@ -595,12 +607,20 @@ extern "C" int crime_mlp_set_data(crime_mlp_t *h, int64_t n, const float *x, con
   drop_graph(h);
   if (h->Xall) MLP_HIP(h, hipFree(h->Xall));
   if (h->Tall) MLP_HIP(h, hipFree(h->Tall));
-  h->Xall = h->Tall = nullptr;
+  if (h->Wall) MLP_HIP(h, hipFree(h->Wall));
+  h->Xall = h->Tall = h->Wall = nullptr;
   h->ndata = 0;
   MLP_HIP(h, hipMalloc((void **)&h->Xall, (size_t)n * NIN * sizeof(float)));
   MLP_HIP(h, hipMalloc((void **)&h->Tall, (size_t)n * sizeof(float)));
   MLP_HIP(h, hipMemcpy(h->Xall, x, (size_t)n * NIN * sizeof(float), hipMemcpyHostToDevice));
   MLP_HIP(h, hipMemcpy(h->Tall, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  MLP_HIP(h, hipMalloc((void **)&h->Wall, (size_t)n * sizeof(float)));
+  if (w) {
+    MLP_HIP(h, hipMemcpy(h->Wall, w, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> ones((size_t)n, 1.0f);
+    MLP_HIP(h, hipMemcpy(h->Wall, ones.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  }
   h->ndata = n;
   return 0;
 }
@@ -632,7 +652,8 @@ extern "C" int crime_mlp_train(crime_mlp_t *h, int steps, int batch, float lr0, 
   if (h->exec && h->graph_batch != batch) drop_graph(h);
   if (!h->exec) {
     MLP_HIP(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_gather, dim3(nblk(batch)), dim3(256), 0, h->stream, h->Xall, h->Tall, batch, h->st, h->X, h->T);
+    hipLaunchKernelGGL(k_gather, dim3(nblk(batch)), dim3(256), 0, h->stream, h->Xall, h->Tall, h->Wall, batch, h->st,
+                       h->X, h->T, h->Wt);
     enqueue_forward(h, batch, 1);
     enqueue_backward(h, batch);
     enqueue_adam(h, batch);

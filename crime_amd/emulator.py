@@ -1,7 +1,10 @@
 """Cross-frequency C_l emulator trained on ensembles of this pipeline.
 
-Inputs (Omega_m, sigma_8, n_s, h, b, nu, nu', log10 l) -> output log C_l(nu, nu') of binned angular
-spectra of the HI temperature maps; network and training run on the GPU through include/crime_mlp.h
+Inputs (Omega_m, sigma_8, n_s, h, b, nu, nu', log10 l) -> output asinh(C_l(nu, nu') / s) of binned angular
+spectra of the HI temperature maps, s = 1e-3 of the median auto spectrum: log C_l + log(2/s) wherever
+C_l >> s (every auto spectrum), and still defined where cross spectra of neighbouring channels scatter
+through zero (a cell wider than a channel splits its fixed mass between them, which anticorrelates the
+two).  Network and training run on the GPU through include/crime_mlp.h
 (BF16 MFMA, FP32 accumulation, Adam).  The plain pipeline (GetHI maps + map2alm of include/crime_sht.h)
 produces the training targets and stays the method the emulator is judged against.
 
@@ -62,18 +65,21 @@ class Mlp:
         self._check(self._lib.crime_mlp_forward(self._h, x.shape[0], _fp(x), _fp(y)))
         return y
 
-    def loss_grad(self, x, t):
+    def loss_grad(self, x, t, w=None):
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NIN)
         t = np.ascontiguousarray(t, dtype=np.float32)
+        w = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+        wp = None if w is None else _fp(w)
         g = np.empty(self.nparam, np.float32)
         loss = C.c_double()
-        self._check(self._lib.crime_mlp_loss_grad(self._h, x.shape[0], _fp(x), _fp(t), C.byref(loss), _fp(g)))
+        self._check(self._lib.crime_mlp_loss_grad(self._h, x.shape[0], _fp(x), _fp(t), wp, C.byref(loss), _fp(g)))
         return loss.value, g
 
-    def set_data(self, x, t):
+    def set_data(self, x, t, w=None):
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NIN)
         t = np.ascontiguousarray(t, dtype=np.float32)
-        self._check(self._lib.crime_mlp_set_data(self._h, x.shape[0], _fp(x), _fp(t)))
+        w = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self._lib.crime_mlp_set_data(self._h, x.shape[0], _fp(x), _fp(t), None if w is None else _fp(w)))
 
     def train(self, steps, batch, lr0, total_steps, seed=0, beta1=0.9, beta2=0.999, eps=1e-8):
         loss = np.empty(steps, np.float32)
@@ -185,30 +191,66 @@ def features(cosmos, spec):
     return x.reshape(-1, NIN)
 
 
+def encode(x, dnu):
+    """Fixed affine pre-transform of the raw inputs: nu' -> (nu' - nu) / channel width, so that the pair
+    type (0 for an auto spectrum, 1 for neighbouring channels) is an exact input rather than a 6% change
+    of a frequency."""
+    z = np.array(x, dtype=np.float64, copy=True)
+    z[:, 6] = (z[:, 6] - z[:, 5]) / dnu
+    return z
+
+
 class Emulator:
-    """Standardised inputs/outputs around a trained Mlp."""
+    """Standardised inputs/outputs around a trained Mlp; targets y = asinh(C / scale)."""
 
-    def __init__(self, mlp, x_mean, x_std, y_mean, y_std):
+    def __init__(self, mlp, x_mean, x_std, y_mean, y_std, scale, dnu):
         self.mlp, self.x_mean, self.x_std, self.y_mean, self.y_std = mlp, x_mean, x_std, y_mean, y_std
+        self.scale, self.dnu = scale, dnu
 
-    def predict_log(self, x):
-        return self.mlp.forward((x - self.x_mean) / self.x_std) * self.y_std + self.y_mean
+    def predict_y(self, x):
+        return self.mlp.forward((encode(x, self.dnu) - self.x_mean) / self.x_std) * self.y_std + self.y_mean
+
+    def predict(self, x):
+        """Binned C_l at raw inputs x (n, 8)."""
+        return np.sinh(self.predict_y(x).astype(np.float64)) * self.scale
+
+
+def gaussian_sigma(mean, spec, nseeds):
+    """Cosmic-variance error of seed-averaged binned spectra on the full sky (Gaussian fields):
+    var C_ab = (C_aa C_bb + C_ab^2) / (N_modes nseeds), N_modes = sum over the bin of (2l + 1); the auto
+    spectrum of a cross pair's second channel is taken equal to the first's (neighbouring channels)."""
+    nm = np.array([np.sum(2 * np.arange(lo, hi) + 1.0) for lo, hi in zip(spec.l_edges[:-1], spec.l_edges[1:])])
+    auto_of = {a: i for i, (a, b) in enumerate(spec.pairs) if a == b}
+    sig = np.empty_like(mean)
+    for i, (a, b) in enumerate(spec.pairs):
+        caa = mean[:, auto_of[a], :]
+        sig[:, i, :] = np.sqrt((caa * caa + mean[:, i, :] ** 2) / (nm * nseeds))
+    return sig
 
 
 def train_emulator(cosmos, cls, spec, steps=4000, batch=4096, lr0=2e-3, seed=0, log=None):
-    """Fit log of the seed-averaged binned C_l.  Returns (Emulator, per-step losses)."""
-    x = features(cosmos, spec)
-    y = np.log(cls.mean(axis=1)).reshape(-1)
+    """Fit asinh(C / scale) of the seed-averaged binned C_l by chi^2 (weights 1/sigma_y^2 from the
+    Gaussian cosmic variance of each target).  Returns (Emulator, per-step losses)."""
+    dnu = (spec.nu_hi - spec.nu_lo) / spec.n_nu
+    x = encode(features(cosmos, spec), dnu)
+    mean = cls.mean(axis=1)
+    autos = [i for i, (a, b) in enumerate(spec.pairs) if a == b]
+    scale = 1e-3 * float(np.median(mean[:, autos, :]))
+    y = np.arcsinh(mean / scale)
+    sig_y = gaussian_sigma(mean, spec, cls.shape[1]) / scale / np.sqrt(1.0 + (mean / scale) ** 2)
+    w = 1.0 / sig_y.reshape(-1) ** 2
+    w /= w.mean()
+    y = y.reshape(-1)
     xm, xs = x.mean(axis=0), x.std(axis=0)
     xs[xs == 0] = 1.0
     ym, ys = y.mean(), y.std()
-    mlp = Mlp(max_batch=max(batch, 64))
+    mlp = Mlp(max_batch=max(batch, 65536))
     mlp.set_params(init_params(seed))
-    mlp.set_data((x - xm) / xs, (y - ym) / ys)
+    mlp.set_data((x - xm) / xs, (y - ym) / ys, w)
     losses = mlp.train(steps, batch, lr0, steps, seed=seed)
     if log:
         log(f"trained {steps} steps: loss {losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}")
-    return Emulator(mlp, xm, xs, ym, ys), losses
+    return Emulator(mlp, xm, xs, ym, ys, scale, dnu), losses
 
 
 def init_params(seed):
@@ -231,7 +273,7 @@ def evaluate(emu, cosmos_train, cls_train, cosmos_test, cls_test, spec):
     nseed = cls_test.shape[1]
     truth = cls_test.mean(axis=1)
     sigma = cls_test.std(axis=1, ddof=1)
-    pred = np.exp(emu.predict_log(features(cosmos_test, spec))).reshape(truth.shape)
+    pred = emu.predict(features(cosmos_test, spec)).reshape(truth.shape)
     z = (pred - truth) / sigma
     # no-emulator method: the C_l of the nearest training cosmology (standardised parameter distance)
     span = RANGES[:, 1] - RANGES[:, 0]

@@ -7,8 +7,9 @@
  *   h1 = silu(x W0 + b0),  h_{k+1} = silu(h_k W_k + b_k) (k = 1..4),  y = h5 . w5 + b5.
  * The four 256x256 layers run on BF16 MFMA with FP32 accumulation (their weights and the activations
  * h_k are rounded to BF16 at the GEMM inputs); the 8-input and the 1-output layers run in FP32.
- * Training minimises the mean squared error (1/B) sum_b (y_b - t_b)^2 with Adam on FP32 master
- * weights.  Parameters travel as one FP32 vector in the order
+ * Training minimises the weighted mean squared error (1/B) sum_b w_b (y_b - t_b)^2 (w_b = 1 when no
+ * weights are given; 1/sigma_b^2 makes it a chi^2) with Adam on FP32 master weights.  Parameters
+ * travel as one FP32 vector in the order
  *   W0[8][256], b0[256], W1[256][256], b1[256], ..., W4[256][256], b4[256], w5[256], b5[1]
  * (W[in][out], row-major); crime_mlp_param_count() gives its length.
  *
@@ -47,12 +48,13 @@ int crime_mlp_get_params(crime_mlp_t *h, float *params);
 /* y[i] for host inputs x[i][8], any n >= 1 (run in batches of max_batch). */
 int crime_mlp_forward(crime_mlp_t *h, int64_t n, const float *x, float *y);
 
-/* Loss and its gradient (FP32, parameter order above) for one host batch, n <= max_batch, n % 64 == 0.
- * The weights are not changed. */
-int crime_mlp_loss_grad(crime_mlp_t *h, int n, const float *x, const float *t, double *loss, float *grad);
+/* Loss and its gradient (FP32, parameter order above) for one host batch, n <= max_batch, n % 64 == 0;
+ * w may be NULL (unit weights).  The parameters are not changed. */
+int crime_mlp_loss_grad(crime_mlp_t *h, int n, const float *x, const float *t, const float *w, double *loss,
+                        float *grad);
 
-/* Training set kept in device memory: x[i][8], t[i], n >= 1. */
-int crime_mlp_set_data(crime_mlp_t *h, int64_t n, const float *x, const float *t);
+/* Training set kept in device memory: x[i][8], t[i], optional weights w[i] (NULL: 1), n >= 1. */
+int crime_mlp_set_data(crime_mlp_t *h, int64_t n, const float *x, const float *t, const float *w);
 
 /* `steps` Adam steps on minibatches of `batch` rows (a multiple of 64, <= max_batch) drawn uniformly
  * with replacement by Philox-4x32-10 keyed by `seed` and the step number.  Learning rate

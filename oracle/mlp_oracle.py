@@ -86,8 +86,8 @@ def forward(p, x, bf16=True):
     return y, zs, hs
 
 
-def loss_grad(p, x, t, bf16=True):
-    """(loss, flat gradient) of (1/B) sum (y - t)^2."""
+def loss_grad(p, x, t, bf16=True, w=None):
+    """(loss, flat gradient) of (1/B) sum w (y - t)^2 (w = 1 by default)."""
     d = unpack(np.asarray(p, np.float64))
     r = to_bf16 if bf16 else (lambda a: a)
     x = np.asarray(x, np.float64)
@@ -95,8 +95,9 @@ def loss_grad(p, x, t, bf16=True):
     n = x.shape[0]
     y, zs, hs = forward(p, x, bf16)
     e = y - t
-    loss = float((e * e).sum() / n)
-    dy = 2.0 * e / n
+    w = np.ones(n) if w is None else np.asarray(w, np.float64)
+    loss = float((w * e * e).sum() / n)
+    dy = 2.0 * w * e / n
     g = {"w5": hs[-1].T @ dy, "b5": np.array([dy.sum()])}
     dz = r(dy[:, None] * d["w5"][None, :] * _dsilu(zs[-1]))
     for k in range(NMID, 0, -1):
