@@ -136,7 +136,7 @@ def bench_ensemble(args):
 
 def bench_emulator(args):
     """Emulator C_l err/sigma: 512 Latin-hypercube cosmologies x 8 realisations of 256^3 / Nside 256 /
-    64-channel mocks train the network; 64 held-out cosmologies x 8 fresh realisations judge it."""
+    64-channel mocks train the network; 64 held-out cosmologies x 16 fresh realisations judge it."""
     import crime_amd.emulator as em
 
     def log(msg):
@@ -147,11 +147,11 @@ def bench_emulator(args):
     test_c = em.latin_hypercube(64, seed=2)
     t0 = time.perf_counter()
     cls_tr, t_mock = em.simulate(train_c, 8, spec, seed0=100000, log=log)
-    cls_te, _ = em.simulate(test_c, 8, spec, seed0=900000, log=log)
+    cls_te, _ = em.simulate(test_c, 16, spec, seed0=900000, log=log)
     t_sim = time.perf_counter() - t0
-    log(f"simulated {len(train_c) * 8 + len(test_c) * 8} mocks in {t_sim:.1f} s ({t_mock * 1e3:.2f} ms/mock)")
+    log(f"simulated {len(train_c) * 8 + len(test_c) * 16} mocks in {t_sim:.1f} s ({t_mock * 1e3:.2f} ms/mock)")
     t1 = time.perf_counter()
-    emu, losses = em.train_emulator(train_c, cls_tr, spec, steps=args.emu_steps, batch=2048, lr0=2e-3, seed=0,
+    emu, losses = em.train_emulator(train_c, cls_tr, spec, steps=args.emu_steps, batch=4096, lr0=2e-3, seed=0,
                                     log=log)
     t_train = time.perf_counter() - t1
     res = em.evaluate(emu, train_c, cls_tr, test_c, cls_te, spec)
@@ -168,10 +168,11 @@ def bench_emulator(args):
         "n_gpus": 1, "steps": args.emu_steps, "warmup": 0, "ms_per_step": t_train * 1e3 / args.emu_steps,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic (mocks of this pipeline on Latin-hypercube cosmologies)",
-        "config": {"workload": "emulator", "train": "512 cosmologies x 8 realisations", "test": "64 x 8",
+        "config": {"workload": "emulator", "train": "512 cosmologies x 8 realisations", "test": "64 x 16",
                    "mock": "256^3, Nside 256, 64 channels 700-800 MHz", "network": "8-256x5-1 SiLU",
                    "pairs": len(spec.pairs), "l_bins": spec.nbins},
         "emulator": res["emulator"], "nearest_cosmology": res["nearest_cosmology"],
+        "sigma_calibration": res["sigma_calibration"],
         "gain_vs_nearest_cosmology": res["nearest_cosmology"]["rms_debiased"] / max(res["emulator"]["rms_debiased"], 1e-9),
         "ms_per_mock": t_mock * 1e3, "us_per_emulated_cosmology": t_eval * 1e6,
         "emulator_rate_vs_mock_rate": t_mock / t_eval,

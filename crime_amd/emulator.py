@@ -200,19 +200,37 @@ def encode(x, dnu):
     return z
 
 
+def fold_standardisation(p, x_mean, x_std, y_mean, y_std):
+    """Parameters of the same network acting on encoded raw inputs and returning y in target units:
+    W0 <- W0 / x_std, b0 <- b0 - (x_mean / x_std) W0, w5 <- y_std w5, b5 <- y_std b5 + y_mean."""
+    q = np.array(p, dtype=np.float64, copy=True)
+    wd = 256
+    w0 = q[:NIN * wd].reshape(NIN, wd)
+    b0 = q[NIN * wd:NIN * wd + wd]
+    b0 -= (x_mean / x_std) @ w0
+    w0 /= x_std[:, None]
+    o5 = q.size - wd - 1
+    q[o5:o5 + wd] *= y_std
+    q[-1] = q[-1] * y_std + y_mean
+    return q.astype(np.float32)
+
+
 class Emulator:
-    """Standardised inputs/outputs around a trained Mlp; targets y = asinh(C / scale)."""
+    """A trained Mlp whose parameters have the input and output standardisation folded in; targets
+    y = asinh(C / scale)."""
 
     def __init__(self, mlp, x_mean, x_std, y_mean, y_std, scale, dnu):
-        self.mlp, self.x_mean, self.x_std, self.y_mean, self.y_std = mlp, x_mean, x_std, y_mean, y_std
-        self.scale, self.dnu = scale, dnu
+        self.mlp, self.scale, self.dnu = mlp, scale, dnu
+        mlp.set_params(fold_standardisation(mlp.params(), x_mean, x_std, y_mean, y_std))
 
     def predict_y(self, x):
-        return self.mlp.forward((encode(x, self.dnu) - self.x_mean) / self.x_std) * self.y_std + self.y_mean
+        z = np.array(x, dtype=np.float32, copy=True)
+        z[:, 6] = (z[:, 6] - z[:, 5]) * np.float32(1.0 / self.dnu)
+        return self.mlp.forward(z)
 
     def predict(self, x):
         """Binned C_l at raw inputs x (n, 8)."""
-        return np.sinh(self.predict_y(x).astype(np.float64)) * self.scale
+        return np.sinh(self.predict_y(x)) * np.float32(self.scale)
 
 
 def gaussian_sigma(mean, spec, nseeds):
@@ -267,12 +285,17 @@ def init_params(seed):
 def evaluate(emu, cosmos_train, cls_train, cosmos_test, cls_test, spec):
     """err/sigma of the emulator and of the nearest-training-cosmology lookup on held-out cosmologies.
 
-    truth = mean of the held-out mocks; sigma = scatter of one mock's binned C_l across seeds (cosmic
-    variance of a realisation).  The truth itself carries sigma / sqrt(nseeds) of noise; the debiased
-    RMS subtracts it in quadrature."""
+    truth = mean of the held-out cosmology's nseeds mocks.  sigma = cosmic variance of one realisation:
+    the Gaussian sigma of each binned spectrum (gaussian_sigma with one realisation) times a factor per
+    (pair, l bin) that calibrates it on the held-out mocks, pooled over all test cosmologies
+    (sum of sample variances / sum of Gaussian variances; non-Gaussianity of the lognormal field raises
+    it above 1).  The truth itself carries sigma / sqrt(nseeds) of noise; the debiased RMS subtracts
+    it in quadrature."""
     nseed = cls_test.shape[1]
     truth = cls_test.mean(axis=1)
-    sigma = cls_test.std(axis=1, ddof=1)
+    sig_g = gaussian_sigma(truth, spec, 1)
+    calib = np.sqrt(cls_test.var(axis=1, ddof=1).sum(axis=0) / (sig_g ** 2).sum(axis=0))
+    sigma = sig_g * calib[None]
     pred = emu.predict(features(cosmos_test, spec)).reshape(truth.shape)
     z = (pred - truth) / sigma
     # no-emulator method: the C_l of the nearest training cosmology (standardised parameter distance)
@@ -286,4 +309,6 @@ def evaluate(emu, cosmos_train, cls_train, cosmos_test, cls_test, spec):
         return {"rms": rms, "rms_debiased": float(np.sqrt(max(0.0, rms ** 2 - 1.0 / nseed))),
                 "median_abs": float(np.median(np.abs(zz))), "p95_abs": float(np.percentile(np.abs(zz), 95))}
 
-    return {"emulator": summary(z), "nearest_cosmology": summary(zn), "truth_noise_sigma": 1.0 / math.sqrt(nseed)}
+    return {"emulator": summary(z), "nearest_cosmology": summary(zn), "truth_noise_sigma": 1.0 / math.sqrt(nseed),
+            "sigma_calibration": {"min": float(calib.min()), "median": float(np.median(calib)),
+                                  "max": float(calib.max())}}

@@ -79,3 +79,19 @@ def test_adam_vs_torch():
         sch.step()
         p, m, v = mo.adam_step(p, m, v, g, step, lr0, total)
         assert np.allclose(p, w.detach().numpy(), rtol=0, atol=1e-12)
+
+
+def test_fold_standardisation():
+    """The emulator folds its input/output standardisation into the first and last layers: the folded
+    network on raw inputs equals the original on standardised inputs, de-standardised."""
+    from crime_amd.emulator import fold_standardisation
+
+    p = mo.init_params(11)
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(100, 8)) * 3 + 5
+    xm, xs = x.mean(axis=0), x.std(axis=0)
+    ym, ys = 1.7, 0.3
+    q = fold_standardisation(p, xm, xs, ym, ys)
+    y0, _, _ = mo.forward(p, (x - xm) / xs, bf16=False)
+    y1, _, _ = mo.forward(q, x, bf16=False)
+    assert np.allclose(y1, y0 * ys + ym, rtol=1e-5, atol=1e-5)
