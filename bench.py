@@ -88,6 +88,16 @@ def bench_ensemble(args):
         hs = [crime_amd.GetHI(p) for p in pars]
         t_create = time.perf_counter() - t0
         cells = sum(h.n ** 3 for h in hs)
+        # one autotune per member kind (grid, Nside); its siblings take the schedule
+        schedules = {}
+        t0 = time.perf_counter()
+        if not args.no_tune:
+            for h in hs:
+                key = (h.n, h.n_side)
+                if key not in schedules:
+                    schedules[key] = h.autotune(reps=2)[0]
+                h.set_schedule(schedules[key])
+        t_tune = time.perf_counter() - t0
         for h in hs:
             h.run()
         # plain per-mock path on resident handles: one crime_gh_run (with its own sync) per member
@@ -130,6 +140,7 @@ def bench_ensemble(args):
         "speedup_vs_plain_end_to_end": (ms_e2e * nmem) / ms,
         "cells_per_s": cells / (ms * 1e-3),
         "host_setup_s_16_threads": t_host, "device_setup_s": t_create,
+        "autotune": {f"{k[0]}^3/Nside{k[1]}": v for k, v in schedules.items()}, "autotune_s": t_tune,
     }
     print(json.dumps(out), flush=True)
 
@@ -188,6 +199,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="production_1024", choices=WORKLOADS + ("ensemble_96", "emulator"))
     ap.add_argument("--emu-steps", type=int, default=20000, help="training steps of the emulator workload")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default kernel schedule (no autotune)")
     ap.add_argument("--streams", type=int, default=4, help="concurrent streams of the ensemble graph")
     ap.add_argument("--cpu-n", type=int, default=768, help="grid of the CPU-baseline sample (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -224,6 +236,12 @@ def main():
         print(f"bench: {args.workload} N={n} Nside={d.n_side} n_nu={d.n_nu} n_sub={d.n_sub} "
               f"prec={'fp64' if prec else 'fp32'}", file=sys.stderr, flush=True)
         gh = crime_amd.GetHI(par)
+        tune = None
+        if not args.no_tune:
+            t0 = time.perf_counter()
+            sched, table = gh.autotune(reps=2)
+            tune = {"schedule": sched, "ms_per_variant": table, "seconds": time.perf_counter() - t0}
+            print(f"bench: autotuned schedule {sched} in {tune['seconds']:.1f} s", file=sys.stderr, flush=True)
         for i in range(args.warmup):
             gh.run()
             print(f"bench: warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
@@ -281,6 +299,7 @@ def main():
                          "bytes_per_launch": by[dom], "ms_per_launch": launch[dom]},
             "cpu_baseline": cpu,
             "stages": stages,
+            "autotune": tune,
             "ms_maps_d2h": ms_d2h,
             "cells_per_s_with_d2h": cells / ((ms + ms_d2h) * 1e-3),
             "maps_sum_mK": binned,

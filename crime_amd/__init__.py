@@ -170,6 +170,27 @@ class GetHI:
             ipix.ctypes.data_as(C.POINTER(C.c_int64))))
         return inu, ipix
 
+    STAGES = ("fill_fft_z", "fft_y", "fft_x", "pixelize")
+
+    def schedule(self):
+        """Kernel variants [fill+kz, ky, kx, pixelize] (crime_gh_get_schedule)."""
+        s = (C.c_int * 4)()
+        self._check(self._lib.crime_gh_get_schedule(self._h, s))
+        return list(s)
+
+    def set_schedule(self, sched):
+        s = (C.c_int * 4)(*[int(v) for v in sched])
+        self._check(self._lib.crime_gh_set_schedule(self._h, s))
+
+    def autotune(self, reps=3):
+        """Time every variant of every stage on this handle and keep the fastest (crime_gh_autotune).
+        Returns (schedule, {stage: [ms of variant 0, 1, 2]}); -1 marks a variant the grid lacks."""
+        s = (C.c_int * 4)()
+        ms = (C.c_double * 12)()
+        self._check(self._lib.crime_gh_autotune(self._h, int(reps), s, ms))
+        table = {st: [ms[3 * i + v] for v in range(3)] for i, st in enumerate(self.STAGES)}
+        return list(s), table
+
     def power_spectrum(self, nbins=None, bin_dk=None):
         """Shell-averaged P(k) of the resident Gaussian delta: (k_mean [h/Mpc], P [(Mpc/h)^3], n_modes).
         Default shells are one fundamental mode wide up to the Nyquist k.  Consumes the fields."""

@@ -148,6 +148,12 @@ def gpu_lib():
         lib.crime_gh_sample_indices.restype = C.c_int
         lib.crime_gh_sample_indices.argtypes = [H, C.c_int64, C.c_int64, C.POINTER(C.c_int32),
                                                 C.POINTER(C.c_int64)]
+        lib.crime_gh_get_schedule.restype = C.c_int
+        lib.crime_gh_get_schedule.argtypes = [H, C.POINTER(C.c_int)]
+        lib.crime_gh_set_schedule.restype = C.c_int
+        lib.crime_gh_set_schedule.argtypes = [H, C.POINTER(C.c_int)]
+        lib.crime_gh_autotune.restype = C.c_int
+        lib.crime_gh_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), _dp]
         lib.crime_gh_power_spectrum.restype = C.c_int
         lib.crime_gh_power_spectrum.argtypes = [H, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int64)]
         lib.crime_gh_device_maps.restype = C.c_int

@@ -42,6 +42,7 @@ struct crime_gh {
   MapParams map{};
   double norm = 0;
   bool fields_done = false;
+  Schedule sched{};            // kernel variants (crime_gh_autotune / crime_gh_set_schedule)
   hipEvent_t ev[8] = {};
   std::string err;
 };
@@ -440,10 +441,10 @@ static int enqueue_fields(crime_gh *h, bool timed, hipStream_t s = nullptr)
   if (!s) s = h->stream;
   if (h->prec)
     e = launch_fields<double>(h->n, (cx<double> *)h->F, h->fill, (const cx<double> *)h->d_tw, h->norm,
-                              h->d_partials, h->d_sig, s, timed ? h->ev : nullptr);
+                              h->d_partials, h->d_sig, s, timed ? h->ev : nullptr, h->sched);
   else
     e = launch_fields<float>(h->n, (cx<float> *)h->F, h->fill, (const cx<float> *)h->d_tw, h->norm,
-                             h->d_partials, h->d_sig, s, timed ? h->ev : nullptr);
+                             h->d_partials, h->d_sig, s, timed ? h->ev : nullptr, h->sched);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "field kernels: %s", hipGetErrorString(e));
   return 0;
 }
@@ -454,10 +455,10 @@ static int enqueue_maps(crime_gh *h, bool timed, hipStream_t s = nullptr)
   if (!s) s = h->stream;
   if (h->prec)
     e = launch_maps<double>((const cx<double> *)h->F, (double *)h->maps, h->nmaps, h->map, s,
-                            timed ? h->ev + 5 : nullptr);
+                            timed ? h->ev + 5 : nullptr, h->sched);
   else
     e = launch_maps<float>((const cx<float> *)h->F, (float *)h->maps, h->nmaps, h->map, s,
-                           timed ? h->ev + 5 : nullptr);
+                           timed ? h->ev + 5 : nullptr, h->sched);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "map kernels: %s", hipGetErrorString(e));
   return 0;
 }
@@ -553,6 +554,90 @@ extern "C" int crime_gh_power_spectrum(crime_gh_t *h, int nbins, double bin_dk, 
     k_mean[i] = c > 0 ? hacc[3 * i + 1] / c * dk : 0.0;
     pk[i] = c > 0 ? hacc[3 * i] / c : 0.0;
   }
+  return 0;
+}
+
+extern "C" int crime_gh_get_schedule(const crime_gh_t *h, int *sched)
+{
+  if (!h || !sched) return CRIME_GH_EINVAL;
+  sched[0] = h->sched.fill;
+  sched[1] = h->sched.y;
+  sched[2] = h->sched.x;
+  sched[3] = h->sched.pix;
+  return 0;
+}
+
+extern "C" int crime_gh_set_schedule(crime_gh_t *h, const int *sched)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (!sched) return set_error(h, CRIME_GH_EINVAL, "schedule is NULL");
+  for (int i = 0; i < 4; i++)
+    if (sched[i] < 0 || sched[i] >= N_SCHED) return set_error(h, CRIME_GH_EINVAL, "schedule[%d] = %d", i, sched[i]);
+  const bool tuned = h->n >= TUNE_MIN_N;
+  h->sched.fill = tuned ? sched[0] : 0;
+  h->sched.y = tuned ? sched[1] : 0;
+  h->sched.x = tuned ? sched[2] : 0;
+  h->sched.pix = sched[3];
+  return 0;
+}
+
+// Runs every variant of each stage `reps` times on the handle's own buffers and keeps the fastest
+// (minimum over the repetitions).  Stages are tuned in order, each with the winners of the ones
+// before it; the field stage is rerun before the map variants.  Leaves fields and maps computed.
+extern "C" int crime_gh_autotune(crime_gh_t *h, int reps, int *sched_out, double *ms_out)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (reps < 1) return set_error(h, CRIME_GH_EINVAL, "reps must be >= 1");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  double table[4][N_SCHED];
+  for (int i = 0; i < 4; i++)
+    for (int v = 0; v < N_SCHED; v++) table[i][v] = -1;
+  const bool tuned = h->n >= TUNE_MIN_N;
+  int *field_slot[3] = {&h->sched.fill, &h->sched.y, &h->sched.x};
+  for (int stage = 0; stage < 3; stage++) {
+    int best = 0;
+    for (int v = 0; v < (tuned ? N_SCHED : 1); v++) {
+      *field_slot[stage] = v;
+      double t = 1e300;
+      for (int r = 0; r < reps; r++) {
+        int rc = enqueue_fields(h, true);
+        if (rc) return rc;
+        HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+        double ms = ev_ms(h->ev[stage], h->ev[stage + 1]);
+        if (ms >= 0 && ms < t) t = ms;
+      }
+      table[stage][v] = t;
+      if (t < table[stage][best]) best = v;
+    }
+    *field_slot[stage] = best;
+  }
+  int rc = enqueue_fields(h, true);
+  if (rc) return rc;
+  rc = finish(h, nullptr, true, false);
+  if (rc) return rc;
+  h->fields_done = true;
+  int best = 0;
+  for (int v = 0; v < N_SCHED; v++) {
+    h->sched.pix = v;
+    double t = 1e300;
+    for (int r = 0; r < reps; r++) {
+      rc = enqueue_maps(h, true);
+      if (rc) return rc;
+      HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+      double ms = ev_ms(h->ev[6], h->ev[7]);
+      if (ms >= 0 && ms < t) t = ms;
+    }
+    table[3][v] = t;
+    if (t < table[3][best]) best = v;
+  }
+  h->sched.pix = best;
+  rc = enqueue_maps(h, false);
+  if (rc) return rc;
+  HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+  if (sched_out) crime_gh_get_schedule(h, sched_out);
+  if (ms_out)
+    for (int i = 0; i < 4; i++)
+      for (int v = 0; v < N_SCHED; v++) ms_out[i * N_SCHED + v] = table[i][v];
   return 0;
 }
 

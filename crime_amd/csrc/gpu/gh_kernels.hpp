@@ -32,12 +32,21 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 template <typename T>
 constexpr int emax() { return sizeof(T) == 4 ? 64 : 32; }
 
-// strided (kz, ky) passes: TX adjacent columns per workgroup, TC threads per column
-template <typename T, int N>
+// Schedule variants the autotuner chooses between, per stage (variant 0 is the default).  Grids
+// below TUNE_MIN_N have variant 0 only.
+constexpr int N_SCHED = 3;
+constexpr int TUNE_MIN_N = 128;
+struct Schedule {
+  int fill = 0, y = 0, x = 0, pix = 0;
+};
+
+// strided (kz, ky) passes: TX adjacent columns per workgroup, TC threads per column; variants aim at
+// 16, 8 or 4 columns per tile
+template <typename T, int N, int V = 0>
 struct ColCfg {
   static constexpr int TC = plan_tc(N, cmax(64, N / emax<T>()));
   static constexpr int CB = N * (int)sizeof(cx<T>);
-  static constexpr int TXMAX = cmin(16, cpow2_div(N));
+  static constexpr int TXMAX = cmin(V == 0 ? 16 : (V == 1 ? 8 : 4), cpow2_div(N));
   static constexpr int tx_for(int budget)
   {
     int tx = TXMAX;
@@ -52,16 +61,18 @@ struct ColCfg {
 };
 
 // the fused fill + kz pass only writes (64-byte write requests waste nothing): a tile of at most
-// 64 KB keeps two workgroups per CU, so one's mode generation overlaps the other's FFT
-template <typename T, int N>
+// 64 KB keeps two workgroups per CU, so one's mode generation overlaps the other's FFT; the
+// variants take 32 KB or 128 KB tiles instead
+template <typename T, int N, int V = 0>
 struct FillCfg {
   static constexpr int TC = ColCfg<T, N>::TC;
-  static constexpr int TX = cmin(ColCfg<T, N>::tx_for(65536), 1024 / TC);
+  static constexpr int TX = cmin(ColCfg<T, N>::tx_for(V == 0 ? 65536 : (V == 1 ? 32768 : 131072)), 1024 / TC);
   static constexpr int NT = TX * TC;
 };
 
-// contiguous kx pass: TY rows per workgroup, TC threads per row
-template <typename T, int N>
+// contiguous kx pass: TY rows per workgroup, TC threads per row; the variants take half or twice the
+// rows (when LDS, threads and N^2 allow)
+template <typename T, int N, int V = 0>
 struct RowCfg {
   static constexpr int TC = plan_tc(N, cmax(64, N / emax<T>()));
   static constexpr int RB = N * (int)sizeof(cx<T>);
@@ -71,7 +82,9 @@ struct RowCfg {
     while (ty > 1 && ty * RB > budget) ty /= 2;
     return ty;
   }
-  static constexpr int TY = ty_for(65536) * RB <= 65536 ? ty_for(65536) : ty_for(131072);
+  static constexpr int TY0 = ty_for(65536) * RB <= 65536 ? ty_for(65536) : ty_for(131072);
+  static constexpr bool CAN2 = 2 * TY0 * RB <= 131072 && 2 * TY0 * TC <= 1024 && (N * N) % (2 * TY0) == 0;
+  static constexpr int TY = V == 1 ? cmax(TY0 / 2, 1) : (V == 2 && CAN2 ? 2 * TY0 : TY0);
   static constexpr int NT = TY * TC;
 };
 
@@ -132,7 +145,7 @@ constexpr int AGG_SLOTS = 2048;
 
 template <typename T>
 hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
-                         double *partials, double *sig, hipStream_t s, hipEvent_t *ev);
+                         double *partials, double *sig, hipStream_t s, hipEvent_t *ev, const Schedule &sc);
 
 constexpr int PK_MAX_BINS = 2048;
 template <typename T>
@@ -142,7 +155,7 @@ int fields_partials_count(int n, int prec);
 
 template <typename T>
 hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m, hipStream_t s,
-                       hipEvent_t *ev);
+                       hipEvent_t *ev, const Schedule &sc);
 template <typename T>
 hipError_t launch_cell_field(const cx<T> *F, const MapParams &m, int which, double *out, hipStream_t s);
 template <typename T>

@@ -129,11 +129,11 @@ __device__ __forceinline__ cx<T> fill_element(const FillParams &f, const AmpEntr
 // ---------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------
-template <typename T, int N>
-__global__ __launch_bounds__((FillCfg<T, N>::NT)) void k_fill_z(cx<T> *__restrict__ F, FillParams f,
-                                                              const cx<T> *__restrict__ tw)
+template <typename T, int N, int V>
+__global__ __launch_bounds__((FillCfg<T, N, V>::NT)) void k_fill_z(cx<T> *__restrict__ F, FillParams f,
+                                                                 const cx<T> *__restrict__ tw)
 {
-  using C = FillCfg<T, N>;
+  using C = FillCfg<T, N, V>;
   __shared__ cx<T> lds[N * C::TX];
   const int c = threadIdx.x % C::TX, t = threadIdx.x / C::TX;
   const int kx = blockIdx.x * C::TX + c, ky = blockIdx.y;
@@ -149,11 +149,11 @@ __global__ __launch_bounds__((FillCfg<T, N>::NT)) void k_fill_z(cx<T> *__restric
   Stockham<T, N, C::TC>::template run<true>(t, load, store, slot, tw);
 }
 
-template <typename T, int N>
-__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_fft_y(cx<T> *__restrict__ F,
-                                                            const cx<T> *__restrict__ tw)
+template <typename T, int N, int V>
+__global__ __launch_bounds__((ColCfg<T, N, V>::NT)) void k_fft_y(cx<T> *__restrict__ F,
+                                                               const cx<T> *__restrict__ tw)
 {
-  using C = ColCfg<T, N>;
+  using C = ColCfg<T, N, V>;
   __shared__ cx<T> lds[N * C::TX];
   const int c = threadIdx.x % C::TX, t = threadIdx.x / C::TX;
   const int kx = blockIdx.x * C::TX + c, kz = blockIdx.y;
@@ -164,13 +164,13 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_fft_y(cx<T> *__restrict_
   Stockham<T, N, C::TC>::run(t, load, store, slot, tw);
 }
 
-template <typename T, int N>
-__global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_fft_x(cx<T> *__restrict__ F,
-                                                            const cx<T> *__restrict__ tw,
-                                                            double norm, double norm_phi,
-                                                            double *__restrict__ partials)
+template <typename T, int N, int V>
+__global__ __launch_bounds__((RowCfg<T, N, V>::NT)) void k_fft_x(cx<T> *__restrict__ F,
+                                                               const cx<T> *__restrict__ tw,
+                                                               double norm, double norm_phi,
+                                                               double *__restrict__ partials)
 {
-  using C = RowCfg<T, N>;
+  using C = RowCfg<T, N, V>;
   __shared__ cx<T> lds[N * C::TY];
   __shared__ double red[2 * C::NT];
   const int t = threadIdx.x % C::TC, b = threadIdx.x / C::TC;
@@ -273,7 +273,7 @@ static hipError_t launch_pk_n(cx<T> *F, const cx<T> *tw, double pscale, double b
   using CC = ColCfg<T, N>;
   using RC = RowCfg<T, N>;
   hipLaunchKernelGGL((k_fft_x_conj<T, N>), dim3((N * N) / RC::TY), dim3(RC::NT), 0, s, F, tw);
-  hipLaunchKernelGGL((k_fft_y<T, N>), dim3(N / CC::TX, N), dim3(CC::NT), 0, s, F, tw);
+  hipLaunchKernelGGL((k_fft_y<T, N, 0>), dim3(N / CC::TX, N), dim3(CC::NT), 0, s, F, tw);
   hipLaunchKernelGGL((k_fft_z<T, N>), dim3(N / CC::TX, N), dim3(CC::NT), 0, s, F, tw);
   size_t n3 = (size_t)N * N * N;
   unsigned nb = (unsigned)((n3 + 255) / 256 < 8192 ? (n3 + 255) / 256 : 8192);
@@ -327,22 +327,49 @@ __global__ __launch_bounds__(1024) void k_sigma_finalize(const double *__restric
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
+template <typename T, int N, int V>
+static void launch_fill_v(cx<T> *F, const FillParams &f, const cx<T> *tw, hipStream_t s)
+{
+  using FC = FillCfg<T, N, V>;
+  hipLaunchKernelGGL((k_fill_z<T, N, V>), dim3(N / FC::TX, N), dim3(FC::NT), 0, s, F, f, tw);
+}
+
+template <typename T, int N, int V>
+static void launch_y_v(cx<T> *F, const cx<T> *tw, hipStream_t s)
+{
+  using CC = ColCfg<T, N, V>;
+  hipLaunchKernelGGL((k_fft_y<T, N, V>), dim3(N / CC::TX, N), dim3(CC::NT), 0, s, F, tw);
+}
+
+template <typename T, int N, int V>
+static int launch_x_v(cx<T> *F, const cx<T> *tw, double norm, double norm_phi, double *partials, hipStream_t s)
+{
+  using RC = RowCfg<T, N, V>;
+  const int nrowblk = (N * N) / RC::TY;
+  hipLaunchKernelGGL((k_fft_x<T, N, V>), dim3(nrowblk), dim3(RC::NT), 0, s, F, tw, norm, norm_phi, partials);
+  return nrowblk;
+}
+
 template <typename T, int N>
 static hipError_t launch_fields_n(cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
-                                  double *partials, double *sig, hipStream_t s, hipEvent_t *ev)
+                                  double *partials, double *sig, hipStream_t s, hipEvent_t *ev,
+                                  const Schedule &sc)
 {
-  using CC = ColCfg<T, N>;
-  using FC = FillCfg<T, N>;
-  using RC = RowCfg<T, N>;
-  dim3 gcol(N / CC::TX, N);
+  constexpr bool tuned = N >= TUNE_MIN_N;
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL((k_fill_z<T, N>), dim3(N / FC::TX, N), dim3(FC::NT), 0, s, F, f, tw);
+  if (tuned && sc.fill == 1) launch_fill_v<T, N, tuned ? 1 : 0>(F, f, tw, s);
+  else if (tuned && sc.fill == 2) launch_fill_v<T, N, tuned ? 2 : 0>(F, f, tw, s);
+  else launch_fill_v<T, N, 0>(F, f, tw, s);
   if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL((k_fft_y<T, N>), gcol, dim3(CC::NT), 0, s, F, tw);
+  if (tuned && sc.y == 1) launch_y_v<T, N, tuned ? 1 : 0>(F, tw, s);
+  else if (tuned && sc.y == 2) launch_y_v<T, N, tuned ? 2 : 0>(F, tw, s);
+  else launch_y_v<T, N, 0>(F, tw, s);
   if (ev) (void)hipEventRecord(ev[2], s);
-  int nrowblk = (N * N) / RC::TY;
-  hipLaunchKernelGGL((k_fft_x<T, N>), dim3(nrowblk), dim3(RC::NT), 0, s, F, tw, norm, norm / f.phi_scale,
-                     partials);
+  const double nphi = norm / f.phi_scale;
+  int nrowblk;
+  if (tuned && sc.x == 1) nrowblk = launch_x_v<T, N, tuned ? 1 : 0>(F, tw, norm, nphi, partials, s);
+  else if (tuned && sc.x == 2) nrowblk = launch_x_v<T, N, tuned ? 2 : 0>(F, tw, norm, nphi, partials, s);
+  else nrowblk = launch_x_v<T, N, 0>(F, tw, norm, nphi, partials, s);
   if (ev) (void)hipEventRecord(ev[3], s);
   hipLaunchKernelGGL(k_sigma_finalize, dim3(1), dim3(1024), 0, s, partials, nrowblk,
                      (double)N * N * N, sig);
@@ -352,11 +379,11 @@ static hipError_t launch_fields_n(cx<T> *F, const FillParams &f, const cx<T> *tw
 
 template <typename T>
 hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
-                         double *partials, double *sig, hipStream_t s, hipEvent_t *ev)
+                         double *partials, double *sig, hipStream_t s, hipEvent_t *ev, const Schedule &sc)
 {
   switch (n) {
 #define CRIME_CASE(NN) \
-  case NN: return launch_fields_n<T, NN>(F, f, tw, norm, partials, sig, s, ev);
+  case NN: return launch_fields_n<T, NN>(F, f, tw, norm, partials, sig, s, ev, sc);
     CRIME_FOR_EACH_N(CRIME_CASE)
 #undef CRIME_CASE
   default: return hipErrorInvalidValue;
@@ -364,9 +391,9 @@ hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, 
 }
 
 template hipError_t launch_fields<float>(int, cx<float> *, const FillParams &, const cx<float> *, double,
-                                         double *, double *, hipStream_t, hipEvent_t *);
+                                         double *, double *, hipStream_t, hipEvent_t *, const Schedule &);
 template hipError_t launch_fields<double>(int, cx<double> *, const FillParams &, const cx<double> *,
-                                          double, double *, double *, hipStream_t, hipEvent_t *);
+                                          double, double *, double *, hipStream_t, hipEvent_t *, const Schedule &);
 
 template <typename T>
 hipError_t launch_amp_table(const FillParams &f, void *tab, hipStream_t s)
@@ -378,11 +405,12 @@ hipError_t launch_amp_table(const FillParams &f, void *tab, hipStream_t s)
 template hipError_t launch_amp_table<float>(const FillParams &, void *, hipStream_t);
 template hipError_t launch_amp_table<double>(const FillParams &, void *, hipStream_t);
 
+// the most row blocks any schedule variant launches (variant 1 has the fewest rows per block)
 int fields_partials_count(int n, int prec)
 {
   switch (n) {
 #define CRIME_CASE(NN) \
-  case NN: return prec ? (NN * NN) / RowCfg<double, NN>::TY : (NN * NN) / RowCfg<float, NN>::TY;
+  case NN: return prec ? (NN * NN) / RowCfg<double, NN, 1>::TY : (NN * NN) / RowCfg<float, NN, 1>::TY;
     CRIME_FOR_EACH_N(CRIME_CASE)
 #undef CRIME_CASE
   default: return 0;

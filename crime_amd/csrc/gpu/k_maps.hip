@@ -110,13 +110,13 @@ __device__ __forceinline__ bool cell_may_bin(const MapParams &m, const CellState
 
 // Add val to bin `key` of the workgroup's LDS table (open addressing, linear probing); a sample that
 // finds no slot within the probe budget goes straight to the map in HBM.
-template <typename T>
+template <typename T, int SLOTS>
 __device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned long long *keys, T *vals,
                                         T *__restrict__ maps)
 {
   const unsigned long long EMPTY = ~0ull;
-  constexpr int SHIFT = 64 - __builtin_ctz(AGG_SLOTS);  // Fibonacci hashing onto the table
-  unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> SHIFT) & (AGG_SLOTS - 1);
+  constexpr int SHIFT = 64 - __builtin_ctz(SLOTS);  // Fibonacci hashing onto the table
+  unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> SHIFT) & (SLOTS - 1);
   for (int p = 0; p < 64; p++) {
     unsigned long long k = keys[h];
     if (k == EMPTY) {
@@ -130,20 +130,21 @@ __device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned 
       atomicAdd(&vals[h], val);
       return;
     }
-    h = (h + 1) & (AGG_SLOTS - 1);
+    h = (h + 1) & (SLOTS - 1);
   }
   unsafeAtomicAdd(maps + key, val);
 }
 
 // One workgroup = one TILE_X x TILE_Y x TILE_Z block of cells (one cell per thread).  Samples are
 // summed per (channel, pixel) bin in LDS and every occupied bin is added to HBM once, so the
-// global atomics scale with the bins a tile touches instead of with its samples.
-template <typename T>
+// global atomics scale with the bins a tile touches instead of with its samples.  SLOTS (the
+// schedule variant: 2048, 1024 or 4096 bins) trades occupancy against probe length and overflow.
+template <typename T, int SLOTS>
 __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T *__restrict__ maps,
                                                   MapParams m)
 {
-  __shared__ unsigned long long keys[AGG_SLOTS];
-  __shared__ T vals[AGG_SLOTS];
+  __shared__ unsigned long long keys[SLOTS];
+  __shared__ T vals[SLOTS];
   const int n = m.n;
   const int t = threadIdx.x;
   // tiles run in order of distance from the observer, so the workgroups resident at one time cover a
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
     active = cell_may_bin(m, s);
   }
   if (!__syncthreads_or(active)) return;  // no cell of the tile reaches a channel
-  for (int i = t; i < AGG_SLOTS; i += 256) {
+  for (int i = t; i < SLOTS; i += 256) {
     keys[i] = ~0ull;
     vals[i] = (T)0;
   }
@@ -176,12 +177,12 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
       int64_t ipix;
       int inu = sample_bin(m, s, isub, ipix);
       if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix)
-        agg_add<T>((unsigned long long)ipix + (unsigned long long)m.npix * inu, (T)(mass_sub * m.prefac[inu]),
-                   keys, vals, maps);
+        agg_add<T, SLOTS>((unsigned long long)ipix + (unsigned long long)m.npix * inu,
+                          (T)(mass_sub * m.prefac[inu]), keys, vals, maps);
     }
   }
   __syncthreads();
-  for (int i = t; i < AGG_SLOTS; i += 256) {
+  for (int i = t; i < SLOTS; i += 256) {
     unsigned long long k = keys[i];
     if (k != ~0ull) unsafeAtomicAdd(maps + k, vals[i]);
   }
@@ -258,7 +259,7 @@ static inline unsigned nblk(size_t n)
 
 template <typename T>
 hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m, hipStream_t s,
-                       hipEvent_t *ev)
+                       hipEvent_t *ev, const Schedule &sc)
 {
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(nmaps)), dim3(256), 0, s, maps, nmaps);
@@ -266,7 +267,9 @@ hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m
   size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
   unsigned gx = (unsigned)(ntiles < 65536 ? ntiles : 65536);
   unsigned gy = (unsigned)((ntiles + gx - 1) / gx);
-  hipLaunchKernelGGL(k_pixelize<T>, dim3(gx, gy), dim3(256), 0, s, F, maps, m);
+  if (sc.pix == 1) hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS / 2>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
+  else if (sc.pix == 2) hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS * 2>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
+  else hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
   if (ev) (void)hipEventRecord(ev[2], s);
   return hipGetLastError();
 }
@@ -304,7 +307,7 @@ hipError_t launch_grid_set(cx<T> *F, size_t ncell, int which, const double *in, 
 
 #define CRIME_INST(T)                                                                                   \
   template hipError_t launch_maps<T>(const cx<T> *, T *, size_t, const MapParams &, hipStream_t,      \
-                                     hipEvent_t *);                                                     \
+                                     hipEvent_t *, const Schedule &);                                   \
   template hipError_t launch_cell_field<T>(const cx<T> *, const MapParams &, int, double *, hipStream_t); \
   template hipError_t launch_sample_indices<T>(const cx<T> *, const MapParams &, int64_t, int64_t,     \
                                                int32_t *, int64_t *, hipStream_t);                      \

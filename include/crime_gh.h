@@ -182,6 +182,21 @@ int crime_gh_sample_indices(crime_gh_t *h, int64_t cell0, int64_t ncell, int32_t
  * crime_gh_create_d_and_vr_fields must run again before crime_gh_mk_T_maps. */
 int crime_gh_power_spectrum(crime_gh_t *h, int nbins, double bin_dk, double *k_mean, double *pk, int64_t *nmodes);
 
+/* Kernel schedule (no counterpart in the reference): sched[4] = variants of the fill+kz pass, the ky
+ * pass, the kx pass and the map-stage aggregation table, each in [0, 3); 0 is the default.  The three
+ * FFT variants differ only in tiling (bit-identical fields); the map variants in the size of the LDS
+ * aggregation table (2048, 1024, 4096 bins: the atomic order of the FP32 sums changes).  Grids below
+ * N = 128 have one FFT variant (their entries read back as 0). */
+int crime_gh_get_schedule(const crime_gh_t *h, int *sched);
+int crime_gh_set_schedule(crime_gh_t *h, const int *sched);
+
+/* Autotuner: times every variant of every stage `reps` times on this handle's buffers, keeps the
+ * fastest of each (stages in order, each tuned with the winners before it) and sets the schedule.
+ * sched_out[4] (optional) receives it, ms_out[4*3] (optional) the best time of every variant (-1 when
+ * the variant does not exist for this grid).  The handle then holds fields and maps, as after
+ * crime_gh_run.  Batches created afterwards capture the tuned schedule. */
+int crime_gh_autotune(crime_gh_t *h, int reps, int *sched_out, double *ms_out);
+
 /* Device pointer and size of the resident maps (n_nu*12*Nside^2 elements of the grid precision),
  * for callers that hand the maps to further device work without a host round trip. */
 int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes);
