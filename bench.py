@@ -69,7 +69,9 @@ def cpu_baseline(workload, n_cpu, tmp):
 
 def bench_ensemble(args):
     """mocks/hour of the 96-member uneven ensemble (32 x 128^3/Nside 128, 32 x 192^3/Nside 256,
-    32 x 256^3/Nside 256; 64 channels; seeds 4000-4095; Omega_m, sigma_8 drawn per member)."""
+    32 x 256^3/Nside 256; 64 channels; seeds 4000-4095; Omega_m, sigma_8 drawn per member): maps in one
+    captured graph (value), then every member's 64 x 64 cross-frequency C_l, l <= 3 Nside - 1
+    (mocks_per_hour_maps_and_cls)."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
 
@@ -113,6 +115,21 @@ def bench_ensemble(args):
         ms = sum(ts) / len(ts)
         print(f"bench: batched graph {ms:.1f} ms per ensemble", file=sys.stderr, flush=True)
         b.close()
+        # per-member cross-frequency C_l (l <= 3 Nside - 1) of the resident maps, one SHT plan per Nside
+        plans = {ns: crime_amd.Sht(ns, 3 * ns - 1, max_maps=64) for ns in sorted({h.n_side for h in hs})}
+        for h in hs[:2]:
+            plans[h.n_side].cls_of(h, cross=True)
+        # spectra stay in HBM like the maps; stage times summed from the plans' own events
+        t0 = time.perf_counter()
+        stage_ms = {}
+        for h in hs:
+            plans[h.n_side].cls_of(h, cross=True, to_host=False)
+            for k, v in plans[h.n_side].stats().items():
+                stage_ms[k] = stage_ms.get(k, 0.0) + v
+        ms_cls = (time.perf_counter() - t0) * 1e3
+        for p in plans.values():
+            p.close()
+        print(f"bench: per-member cross C_l of {nmem} members in {ms_cls:.1f} ms", file=sys.stderr, flush=True)
         # the drop-in sequence for one mock (what one GetHI run per mock does): host set-up, device
         # set-up, run, maps to the host, release -- timed on 8 members of each kind
         sample = [i for k in range(3) for i in range(32 * k, 32 * k + 8)]
@@ -134,6 +151,8 @@ def bench_ensemble(args):
         "data": "synthetic (Philox modes of BBKS P(k) per member cosmology; resident in HBM)",
         "config": {"workload": "ensemble_96", "members": "32x128^3/Nside128 + 32x192^3/Nside256 + "
                    "32x256^3/Nside256", "n_nu": 64, "cells_per_ensemble": cells, "streams": args.streams},
+        "ms_cls_per_ensemble": ms_cls, "cls_stage_ms": stage_ms,
+        "mocks_per_hour_maps_and_cls": nmem * 3600e3 / (ms + ms_cls),
         "plain_resident_mocks_per_hour": nmem * 3600e3 / ms_plain,
         "plain_end_to_end_mocks_per_hour": 3600e3 / ms_e2e,
         "speedup_vs_plain_resident": ms_plain / ms,

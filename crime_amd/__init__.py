@@ -322,8 +322,15 @@ class Sht:
         self._check(self._lib.crime_sht_anafast(self._p, n, _dptr(m), int(cross), _dptr(out)))
         return out
 
-    def anafast_device(self, dptr, nmaps, fp64, cross=True):
-        """C_l of device maps: (nmaps, nmaps, lmax+1) if cross else (nmaps, lmax+1)."""
+    def anafast_device(self, dptr, nmaps, fp64, cross=True, to_host=True):
+        """C_l of device maps: (nmaps, nmaps, lmax+1) if cross else (nmaps, lmax+1).  to_host=False keeps
+        them in the plan's device buffer and returns (device pointer, bytes)."""
+        if not to_host:
+            self._check(self._lib.crime_sht_anafast_device(self._p, nmaps, C.c_void_p(dptr), int(fp64), int(cross),
+                                                           None))
+            p, nb = C.c_void_p(), C.c_size_t()
+            self._check(self._lib.crime_sht_device_cls(self._p, C.byref(p), C.byref(nb)))
+            return p.value, nb.value
         shape = (nmaps, nmaps, self.lmax + 1) if cross else (nmaps, self.lmax + 1)
         out = np.empty(shape)
         self._check(self._lib.crime_sht_anafast_device(self._p, nmaps, C.c_void_p(dptr), int(fp64), int(cross),
@@ -338,13 +345,13 @@ class Sht:
                                                           pr.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(out)))
         return out
 
-    def cls_of(self, gh, cross=True, channels=None, pairs=None):
+    def cls_of(self, gh, cross=True, channels=None, pairs=None, to_host=True):
         """C_l of a GetHI handle's resident maps: all pairs (cross), autospectra, or the listed pairs."""
         p, _ = gh.device_maps()
         n = gh.n_nu if channels is None else int(channels)
         if pairs is not None:
             return self.cls_pairs_device(p, n, gh.precision == FP64, pairs)
-        return self.anafast_device(p, n, gh.precision == FP64, cross)
+        return self.anafast_device(p, n, gh.precision == FP64, cross, to_host)
 
     def stats(self):
         st = _ffi.CrimeShtStats()

@@ -184,6 +184,8 @@ def gpu_lib():
         lib.crime_sht_cls_pairs_device.argtypes = [H, C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int32), _dp]
         lib.crime_sht_anafast.restype = C.c_int
         lib.crime_sht_anafast.argtypes = [H, C.c_int, _dp, C.c_int, _dp]
+        lib.crime_sht_device_cls.restype = C.c_int
+        lib.crime_sht_device_cls.argtypes = [H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
         lib.crime_sht_get_stats.restype = C.c_int
         lib.crime_sht_get_stats.argtypes = [H, C.POINTER(CrimeShtStats)]
         lib.crime_mlp_param_count.restype = C.c_int64

@@ -5,6 +5,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,8 @@ namespace {
 // otherwise it is rebuilt per call in chunks of m of at most this size
 constexpr size_t LAMBDA_BUDGET = (size_t)16 << 30;
 constexpr size_t LDS_LIMIT = 160 * 1024;
+constexpr int BLU_MIN_N = 32;     // shorter rings: the split DFT
+constexpr int BLU_MAX_M = 8192;   // largest compiled Bluestein convolution
 
 struct Chunk {
   int m0, m1;        // [m0, m1)
@@ -41,9 +44,14 @@ struct crime_sht {
   int4 *d_tiles = nullptr;
   double *d_G = nullptr, *d_alm = nullptr, *d_lam = nullptr;
   double *d_cls = nullptr;
-  size_t cls_cap = 0;
+  size_t cls_cap = 0, cls_bytes = 0;
   double *d_unpack = nullptr;
   double *d_hmaps = nullptr;
+  // ring transforms: groups of rings with one kernel each, their twiddles and Bluestein tables
+  std::vector<ShtLaunch> groups;
+  double *d_tw = nullptr;
+  int64_t *d_woff = nullptr, *d_boff = nullptr;
+  double *d_wtab = nullptr, *d_btab = nullptr;
   std::vector<Chunk> chunks;
   bool resident = false;
   std::vector<hipEvent_t> ev;
@@ -81,7 +89,8 @@ extern "C" void crime_sht_destroy(crime_sht_t *p)
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   void *bufs[] = {p->d_rings, p->d_z, p->d_st, p->d_coef, p->d_off, p->d_tiles, p->d_G,
-                  p->d_alm, p->d_lam, p->d_cls, p->d_unpack, p->d_hmaps};
+                  p->d_alm, p->d_lam, p->d_cls, p->d_unpack, p->d_hmaps, p->d_tw,
+                  p->d_woff, p->d_boff, p->d_wtab, p->d_btab};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
@@ -118,6 +127,116 @@ static void ring_geometry(int64_t nside, std::vector<ShtRing> &rings, std::vecto
   }
 }
 
+// How a ring of n pixels is transformed: the equatorial rings by a Stockham FFT of length 4 Nside
+// where one is compiled, rings of 32 .. 4096 pixels by Bluestein, the rest by the split DFT.
+static int ring_kind(int64_t nside, int n, int *len)
+{
+  const int fft = ring_fft_length(nside);
+  if (fft && n == fft) {
+    if (len) *len = fft;
+    return SHT_RING_FFT;
+  }
+  if (n >= BLU_MIN_N) {
+    int m = 1;
+    while (m < 2 * n - 1) m *= 2;
+    if (m <= BLU_MAX_M) {
+      if (len) *len = m;
+      return SHT_RING_BLUESTEIN;
+    }
+  }
+  if (len) *len = 0;
+  return SHT_RING_SPLIT;
+}
+
+static void push_cis(std::vector<double> &v, long double num, long double den)
+{
+  const long double a = 3.14159265358979323846264338327950288L * num / den;
+  v.push_back((double)cosl(a));
+  v.push_back((double)sinl(a));
+}
+
+// Groups of consecutive rings that share a kernel, twiddle tables exp(+2 pi i m / len) per length,
+// and for the Bluestein rings the chirp w_j = exp(+i pi j^2 / n) (j^2 reduced mod 2n exactly) and
+// the transformed filter B, computed here on the device by the Stockham engine.
+static hipError_t plan_rings(crime_sht *p, const std::vector<ShtRing> &rings)
+{
+  const int R = (int)rings.size();
+  std::vector<int> kind(R), len(R);
+  for (int r = 0; r < R; r++) kind[r] = ring_kind(p->nside, rings[r].nphi, &len[r]);
+
+  std::vector<double> tw;
+  std::vector<std::pair<int, size_t>> tw_off;    // (len, offset in complex)
+  auto twiddles = [&](int L) -> size_t {
+    for (auto &e : tw_off)
+      if (e.first == L) return e.second;
+    const size_t off = tw.size() / 2;
+    for (int m = 0; m < L; m++) push_cis(tw, 2.0L * m, (long double)L);
+    tw_off.push_back({L, off});
+    return off;
+  };
+  std::vector<int64_t> woff(R, 0), boff(R, 0);
+  std::vector<double> wtab;
+  std::vector<std::pair<int, int>> first_of_n;   // (n, ring whose tables every ring of this n shares)
+  int64_t btab = 0;
+  std::vector<size_t> group_tw;
+  for (int r = 0; r < R;) {
+    int e = r + 1;
+    while (e < R && kind[e] == kind[r] && len[e] == len[r]) e++;
+    ShtLaunch g{};
+    g.kind = kind[r];
+    g.len = len[r];
+    g.r0 = r;
+    g.nr = e - r;
+    p->groups.push_back(g);
+    group_tw.push_back(g.kind == SHT_RING_SPLIT ? 0 : twiddles(g.len));
+    for (int q = r; q < e && g.kind == SHT_RING_BLUESTEIN; q++) {
+      const int n = rings[q].nphi;
+      int src = -1;
+      for (auto &f : first_of_n)
+        if (f.first == n) src = f.second;
+      if (src >= 0) {
+        woff[q] = woff[src];
+        boff[q] = boff[src];
+        continue;
+      }
+      first_of_n.push_back({n, q});
+      woff[q] = (int64_t)(wtab.size() / 2);
+      for (int64_t j = 0; j < n; j++) push_cis(wtab, (long double)((j * j) % (2 * (int64_t)n)), (long double)n);
+      boff[q] = btab;
+      btab += g.len;
+    }
+    r = e;
+  }
+  hipError_t err;
+  if ((err = hipMalloc((void **)&p->d_tw, std::max<size_t>(tw.size(), 2) * sizeof(double))) != hipSuccess) return err;
+  if (!tw.empty() &&
+      (err = hipMemcpy(p->d_tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+    return err;
+  for (size_t i = 0; i < p->groups.size(); i++) p->groups[i].tw = p->d_tw + 2 * group_tw[i];
+  if (btab == 0) return hipSuccess;
+  if ((err = hipMalloc((void **)&p->d_woff, R * sizeof(int64_t))) != hipSuccess) return err;
+  if ((err = hipMalloc((void **)&p->d_boff, R * sizeof(int64_t))) != hipSuccess) return err;
+  if ((err = hipMalloc((void **)&p->d_wtab, wtab.size() * sizeof(double))) != hipSuccess) return err;
+  if ((err = hipMalloc((void **)&p->d_btab, (size_t)btab * 2 * sizeof(double))) != hipSuccess) return err;
+  if ((err = hipMemcpy(p->d_woff, woff.data(), R * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess) return err;
+  if ((err = hipMemcpy(p->d_boff, boff.data(), R * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess) return err;
+  if ((err = hipMemcpy(p->d_wtab, wtab.data(), wtab.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+    return err;
+  for (ShtLaunch &g : p->groups) {
+    if (g.kind != SHT_RING_BLUESTEIN) continue;
+    g.woff = p->d_woff;
+    g.wtab = p->d_wtab;
+    g.boff = p->d_boff;
+    g.B = p->d_btab;
+    // rings of one length share their tables: fill them once
+    const bool shared = rings[g.r0].nphi == rings[g.r0 + g.nr - 1].nphi;
+    if ((err = launch_blu_setup(g.len, p->d_rings, g.r0, shared ? 1 : g.nr, p->d_woff, p->d_wtab, p->d_boff,
+                                p->d_btab, g.tw, p->stream)) != hipSuccess)
+      return err;
+  }
+  return hipStreamSynchronize(p->stream);
+}
+
 static int create_impl(crime_sht *p, int64_t nside, int lmax, int max_maps, int device)
 {
   if (nside < 1 || nside > 8192) return sht_error(p, CRIME_GH_EINVAL, "nside %lld outside [1, 8192]", (long long)nside);
@@ -134,7 +253,11 @@ static int create_impl(crime_sht *p, int64_t nside, int lmax, int max_maps, int 
   p->ldg = 2 * max_maps;
   p->ldo = 2 * max_maps;
   p->nalm = (int64_t)(lmax + 1) * (lmax + 2) / 2;
-  p->lds = ring_dft_lds_bytes((int)(4 * nside), lmax);
+  p->lds = 0;
+  for (int64_t i = 1; i <= 2 * nside; i++) {
+    const int n = (int)(4 * (i < nside ? i : nside));
+    if (ring_kind(nside, n, nullptr) == SHT_RING_SPLIT) p->lds = std::max(p->lds, ring_dft_lds_bytes(n, lmax));
+  }
   if (p->lds > LDS_LIMIT)
     return sht_error(p, CRIME_GH_EINVAL, "nside %lld: a ring spectrum needs %zu bytes of LDS (limit %zu)",
                      (long long)nside, p->lds, LDS_LIMIT);
@@ -154,6 +277,8 @@ static int create_impl(crime_sht *p, int64_t nside, int lmax, int max_maps, int 
   SHT_HIP(p, hipMemcpy(p->d_rings, rings.data(), rings.size() * sizeof(ShtRing), hipMemcpyHostToDevice));
   SHT_HIP(p, hipMemcpy(p->d_z, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
   SHT_HIP(p, hipMemcpy(p->d_st, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
+
+  SHT_HIP(p, plan_rings(p, rings));
 
   // recursion coefficients per m: [C_m, sqrt(2m+3), (a_l, b_l) for l = m+2 .. lmax]
   std::vector<int64_t> off(lmax + 2);
@@ -250,8 +375,20 @@ static int analyse(crime_sht *p, int nmaps, const void *d_maps, int fp64)
   if (!d_maps) return sht_error(p, CRIME_GH_EINVAL, "maps are NULL");
   hipEvent_t *ev = p->ev.data();
   SHT_HIP(p, hipEventRecord(ev[0], p->stream));
-  SHT_HIP(p, launch_ring_dft(d_maps, fp64, nmaps, p->npix, p->d_rings, p->R, p->RP, p->lmax, p->ldg,
-                             4.0 * M_PI / (double)p->npix, p->lds, p->d_G, p->stream));
+  for (ShtLaunch g : p->groups) {
+    g.nmaps = nmaps;
+    g.npix = p->npix;
+    g.rings = p->d_rings;
+    g.R = p->R;
+    g.RP = p->RP;
+    g.lmax = p->lmax;
+    g.ldg = p->ldg;
+    g.w = 4.0 * M_PI / (double)p->npix;
+    g.lds = p->lds;
+    g.G = p->d_G;
+    g.s = p->stream;
+    SHT_HIP(p, launch_rings(g, d_maps, fp64));
+  }
   SHT_HIP(p, hipEventRecord(ev[1], p->stream));
   for (size_t i = 0; i < p->chunks.size(); i++) {
     const Chunk &c = p->chunks[i];
@@ -332,7 +469,6 @@ extern "C" int crime_sht_anafast_device(crime_sht_t *p, int nmaps, const void *d
                                         double *cls)
 {
   if (!p) return sht_error(nullptr, CRIME_GH_EINVAL, "plan is NULL");
-  if (!cls) return sht_error(p, CRIME_GH_EINVAL, "cls are NULL");
   SHT_HIP(p, hipSetDevice(p->device));
   int rc = analyse(p, nmaps, d_maps, maps_fp64);
   if (rc) return rc;
@@ -347,7 +483,16 @@ extern "C" int crime_sht_anafast_device(crime_sht_t *p, int nmaps, const void *d
   SHT_HIP(p, launch_cls(p->d_alm, p->lmax, nmaps, p->ldo, cross, p->d_cls, p->stream));
   rc = finish(p, true);
   if (rc) return rc;
-  SHT_HIP(p, hipMemcpy(cls, p->d_cls, n * sizeof(double), hipMemcpyDeviceToHost));
+  p->cls_bytes = n * sizeof(double);
+  if (cls) SHT_HIP(p, hipMemcpy(cls, p->d_cls, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int crime_sht_device_cls(const crime_sht_t *p, const double **dptr, size_t *bytes)
+{
+  if (!p || !dptr || !bytes) return CRIME_GH_EINVAL;
+  *dptr = p->d_cls;
+  *bytes = p->cls_bytes;
   return 0;
 }
 
@@ -375,6 +520,7 @@ extern "C" int crime_sht_cls_pairs_device(crime_sht_t *p, int nmaps, const void 
   SHT_HIP(p, launch_cls_pairs(p->d_alm, p->lmax, p->ldo, npairs, d_pairs, p->d_cls, p->stream));
   rc = finish(p, true);
   if (rc) return rc;
+  p->cls_bytes = n * sizeof(double);
   SHT_HIP(p, hipMemcpy(cls, p->d_cls, n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }

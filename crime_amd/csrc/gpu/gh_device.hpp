@@ -30,6 +30,14 @@ __device__ __forceinline__ cx<T> cmuli(cx<T> a) { return {-a.y, a.x}; }
 template <typename T>
 __device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.x, -a.y}; }
 
+// Workgroups are dealt to the 8 XCDs round robin (b -> XCD b % 8).  This bijection of [0, nb)
+// hands each XCD one contiguous run of logical blocks, so neighbours share that XCD's L2.
+__device__ __forceinline__ int xcd_swizzle(int b, int nb)
+{
+  const int q = nb / 8, rem = nb % 8, x = b % 8;
+  return x * q + (x < rem ? x : rem) + b / 8;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Philox-4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11): counter-based, so mode k can be drawn by
 // any thread in any order and the mode array does not depend on the launch geometry.

@@ -15,8 +15,38 @@ struct ShtRing {
 
 size_t ring_dft_lds_bytes(int n, int lmax);
 hipError_t sht_set_lds_limits(size_t bytes);
-hipError_t launch_ring_dft(const void *maps, int fp64, int nmaps, int64_t npix, const ShtRing *rings, int R, int RP,
-                           int lmax, int ldg, double w, size_t lds, double *G, hipStream_t s);
+// compiled Stockham lengths of the equatorial rings (4 Nside for Nside 32 .. 1024)
+#define CRIME_FOR_EACH_RING_FFT_L(X) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+// 4 Nside when the equatorial rings have a compiled Stockham plan, else 0
+int ring_fft_length(int64_t nside);
+
+// convolution lengths of the Bluestein ring transforms (rings of 32 .. 4096 pixels)
+#define CRIME_FOR_EACH_BLU_M(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
+
+// how a contiguous range of rings is transformed
+enum { SHT_RING_SPLIT = 0, SHT_RING_FFT = 1, SHT_RING_BLUESTEIN = 2 };
+
+struct ShtLaunch {
+  int kind, len;                 // SHT_RING_*; FFT length (4 Nside) or Bluestein M
+  int r0, nr;                    // rings [r0, r0 + nr)
+  int nmaps;
+  int64_t npix;
+  const ShtRing *rings;
+  int R, RP, lmax, ldg;
+  double w;                      // 4 pi / Npix
+  size_t lds;                    // dynamic LDS of the split DFT
+  double *G;
+  const void *tw;                // exp(+2 pi i m / len), m < len
+  const int64_t *woff;           // Bluestein: per-ring offsets of the chirp and filter tables
+  const void *wtab;
+  const int64_t *boff;
+  const void *B;
+  hipStream_t s;
+};
+hipError_t launch_rings(const ShtLaunch &a, const void *maps, int fp64);
+// B of the Bluestein rings [r0, r0 + nr) of one convolution length M
+hipError_t launch_blu_setup(int M, const ShtRing *rings, int r0, int nr, const int64_t *woff, const void *wtab,
+                            const int64_t *boff, void *B, const void *tw, hipStream_t s);
 hipError_t launch_legendre(const double2 *coef, const int64_t *off, int m0, int m1, int lmax, const double *zs,
                            const double *sts, int R, int RP, double *lam, hipStream_t s);
 // tiles: (m, parity, row tile, rows of that parity)

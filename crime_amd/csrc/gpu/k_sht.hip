@@ -10,6 +10,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "gh_fft.hpp"
 #include "gh_sht.hpp"
 
 namespace crime {
@@ -77,10 +78,10 @@ __device__ void ring_spectrum(const TM *__restrict__ f_g, int n, int K, int P, c
 template <typename TM>
 __global__ __launch_bounds__(512) void k_ring_dft(const TM *__restrict__ maps, int64_t npix,
                                                   const ShtRing *__restrict__ rings, int R, int RP, int lmax,
-                                                  int ldg, double w, double *__restrict__ G)
+                                                  int ldg, double w, double *__restrict__ G, int r0)
 {
   extern __shared__ double2 lds_sht[];
-  const int c = blockIdx.x, r = blockIdx.y;
+  const int c = blockIdx.x, r = r0 + blockIdx.y;
   const ShtRing ri = rings[r];
   const int n = ri.nphi;
   const int K = n < lmax + 1 ? n : lmax + 1;
@@ -118,6 +119,182 @@ __global__ __launch_bounds__(512) void k_ring_dft(const TM *__restrict__ maps, i
     ge[1] = w * (er * pi + ei * pr);
     go[0] = w * (orr * pr - oi * pi);
     go[1] = w * (orr * pi + oi * pr);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// equatorial rings (all 4 Nside long) by the Stockham engine: one line per (map, ring pair) packs
+// the north ring and its mirror as z = f_N + i f_S; Z = sum_j z_j e^{+2 pi i jk/L} gives
+// A_N = (Z_k + conj Z_{-k})/2, A_S = (Z_k - conj Z_{-k})/(2i), and the forward spectra D = conj(A).
+// The equator (the last line of a map) has no mirror: f_S = 0 there.
+// ---------------------------------------------------------------------------------------------
+constexpr int sht_imax(int a, int b) { return a > b ? a : b; }
+constexpr int sht_imin(int a, int b) { return a < b ? a : b; }
+
+// threads per line: L / RING_TC_DIV, at least 64
+#ifndef CRIME_RING_TC_DIV
+#define CRIME_RING_TC_DIV 16
+#endif
+
+template <int L>
+struct RingFftCfg {
+  static constexpr int TC = plan_tc(L, sht_imax(64, L / CRIME_RING_TC_DIV));
+  // lines per workgroup: at least 128 threads where 64 KB of LDS allows
+  static constexpr int G = sht_imax(1, sht_imin(sht_imax(4, 128 / TC), 65536 / (16 * L)));
+  static constexpr int NT = G * TC;
+};
+
+template <typename TM, int L>
+__global__ __launch_bounds__(RingFftCfg<L>::NT) void k_ring_fft(const TM *__restrict__ maps, int64_t npix,
+                                                               const ShtRing *__restrict__ rings, int R, int RP,
+                                                               int lmax, int ldg, double w, double *__restrict__ G,
+                                                               int r0, int nmaps, int nlines,
+                                                               const cx<double> *__restrict__ tw)
+{
+  using C = RingFftCfg<L>;
+  __shared__ cx<double> lds[C::G * L];
+  const int g = threadIdx.x / C::TC, t = threadIdx.x % C::TC;
+  const int line = xcd_swizzle(blockIdx.x, gridDim.x) * C::G + g;
+  const bool live = line < nlines;            // padding lines still join every barrier
+  const int c = live ? line % nmaps : 0, r = r0 + (live ? line / nmaps : 0);
+  const ShtRing ri = rings[r];
+  const bool eq = (r == R - 1);
+  const TM *fn = maps + (size_t)c * npix + ri.first_n;
+  const TM *fs = maps + (size_t)c * npix + ri.first_s;
+  cx<double> *buf = lds + g * L;
+  auto load = [&](int e) -> cx<double> {
+    if (!live) return {0.0, 0.0};
+    return {(double)fn[e], eq ? 0.0 : (double)fs[e]};
+  };
+  auto store = [&](int e, cx<double> v) { buf[e] = v; };
+  auto slot = [&](int e) -> cx<double> & { return buf[e]; };
+  Stockham<double, L, C::TC>::run(t, load, store, slot, tw);
+  __syncthreads();
+  if (!live) return;
+  const size_t plane = (size_t)RP * ldg;
+  for (int m = t; m <= lmax; m += C::TC) {
+    const cx<double> zk = buf[m], zm = buf[(L - m) & (L - 1)];
+    // D_N = conj(A_N), D_S = conj(A_S)
+    const double dnr = 0.5 * (zk.x + zm.x), dni = -0.5 * (zk.y - zm.y);
+    const double dsr = 0.5 * (zk.y + zm.y), dsi = 0.5 * (zk.x - zm.x);
+    double pr = 1, pi = 0;
+    if (ri.s) {
+      const int tt = (int)((int64_t)m % (2 * L));
+      sincospi(-(double)tt / (double)L, &pi, &pr);
+    }
+    const double er = dnr + dsr, ei = dni + dsi, orr = dnr - dsr, oi = dni - dsi;
+    double *ge = G + (size_t)(2 * m) * plane + (size_t)r * ldg + 2 * c;
+    double *go = ge + plane;
+    ge[0] = w * (er * pr - ei * pi);
+    ge[1] = w * (er * pi + ei * pr);
+    go[0] = w * (orr * pr - oi * pi);
+    go[1] = w * (orr * pi + oi * pr);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Bluestein (1970) for rings of any length n: with w_j = e^{+i pi j^2 / n} (exact: j^2 mod 2n),
+//   Z_k = sum_j z_j e^{+2 pi i jk/n} = w_k sum_j (z_j w_j) conj(w_{k-j}),
+// a cyclic convolution of length M = 2^a >= 2n - 1 done with the Stockham engine:
+//   A = P(a),  conv = conj(P(conj(A B))),  B = P(b) / M,  b_m = conj(w_m) (m < n), b_{M-m} = conj(w_m)
+// (P: the engine's sign-+ transform of length M).  B and w are tabulated per ring at plan creation.
+// z packs the north ring and its mirror as in k_ring_fft.
+// ---------------------------------------------------------------------------------------------
+template <int M>
+struct BluCfg {
+  static constexpr int TC = plan_tc(M, sht_imax(64, M / CRIME_RING_TC_DIV));
+  static constexpr int G = sht_imax(1, sht_imin(sht_imax(4, 128 / TC), 65536 / (16 * M)));
+  static constexpr int NT = G * TC;
+};
+
+template <int M>
+__global__ __launch_bounds__(BluCfg<M>::NT) void k_blu_setup(const ShtRing *__restrict__ rings, int r0, int nr,
+                                                            const int64_t *__restrict__ woff,
+                                                            const cx<double> *__restrict__ wtab,
+                                                            const int64_t *__restrict__ boff,
+                                                            cx<double> *__restrict__ B,
+                                                            const cx<double> *__restrict__ tw)
+{
+  using C = BluCfg<M>;
+  __shared__ cx<double> lds[C::G * M];
+  const int g = threadIdx.x / C::TC, t = threadIdx.x % C::TC;
+  const int line = xcd_swizzle(blockIdx.x, gridDim.x) * C::G + g;
+  const bool live = line < nr;
+  const int r = r0 + (live ? line : 0);
+  const int n = rings[r].nphi;
+  const cx<double> *w = wtab + woff[r];
+  cx<double> *buf = lds + g * M;
+  auto load = [&](int e) -> cx<double> {
+    if (!live) return {0.0, 0.0};
+    const int m = e < n ? e : (e > M - n ? M - e : -1);
+    if (m < 0) return {0.0, 0.0};
+    return {w[m].x, -w[m].y};
+  };
+  auto store = [&](int e, cx<double> v) {
+    if (live) B[boff[r] + e] = {v.x / (double)M, v.y / (double)M};
+  };
+  auto slot = [&](int e) -> cx<double> & { return buf[e]; };
+  Stockham<double, M, C::TC>::run(t, load, store, slot, tw);
+}
+
+template <typename TM, int M>
+__global__ __launch_bounds__(BluCfg<M>::NT) void k_ring_blu(const TM *__restrict__ maps, int64_t npix,
+                                                           const ShtRing *__restrict__ rings, int R, int RP, int lmax,
+                                                           int ldg, double wn, double *__restrict__ G, int r0,
+                                                           int nmaps, int nlines, const int64_t *__restrict__ woff,
+                                                           const cx<double> *__restrict__ wtab,
+                                                           const int64_t *__restrict__ boff,
+                                                           const cx<double> *__restrict__ B,
+                                                           const cx<double> *__restrict__ tw)
+{
+  using C = BluCfg<M>;
+  __shared__ cx<double> lds[C::G * M];
+  const int g = threadIdx.x / C::TC, t = threadIdx.x % C::TC;
+  const int line = xcd_swizzle(blockIdx.x, gridDim.x) * C::G + g;
+  const bool live = line < nlines;
+  const int c = live ? line % nmaps : 0, r = r0 + (live ? line / nmaps : 0);
+  const ShtRing ri = rings[r];
+  const int n = ri.nphi;
+  const bool eq = (r == R - 1);
+  const TM *fn = maps + (size_t)c * npix + ri.first_n;
+  const TM *fs = maps + (size_t)c * npix + ri.first_s;
+  const cx<double> *w = wtab + woff[r];
+  const cx<double> *Bk = B + boff[r];
+  cx<double> *buf = lds + g * M;
+  auto load1 = [&](int e) -> cx<double> {
+    if (!live || e >= n) return {0.0, 0.0};
+    const cx<double> z = {(double)fn[e], eq ? 0.0 : (double)fs[e]};
+    return cmul(z, w[e]);
+  };
+  auto store1 = [&](int e, cx<double> v) { buf[e] = v; };
+  auto slot = [&](int e) -> cx<double> & { return buf[e]; };
+  Stockham<double, M, C::TC>::run(t, load1, store1, slot, tw);
+  __syncthreads();
+  auto load2 = [&](int e) -> cx<double> { return cconj(cmul(buf[e], Bk[e])); };
+  auto store2 = [&](int e, cx<double> v) {
+    if (e < n) buf[e] = cmul(w[e], cconj(v));
+  };
+  Stockham<double, M, C::TC>::template run<true>(t, load2, store2, slot, tw);
+  __syncthreads();
+  if (!live) return;
+  const size_t plane = (size_t)RP * ldg;
+  for (int m = t; m <= lmax; m += C::TC) {
+    const int k = m % n, km = k == 0 ? 0 : n - k;
+    const cx<double> zk = buf[k], zm = buf[km];
+    const double dnr = 0.5 * (zk.x + zm.x), dni = -0.5 * (zk.y - zm.y);
+    const double dsr = 0.5 * (zk.y + zm.y), dsi = 0.5 * (zk.x - zm.x);
+    double pr = 1, pi = 0;
+    if (ri.s) {
+      const int tt = (int)((int64_t)m % (2 * n));
+      sincospi(-(double)tt / (double)n, &pi, &pr);
+    }
+    const double er = dnr + dsr, ei = dni + dsi, orr = dnr - dsr, oi = dni - dsi;
+    double *ge = G + (size_t)(2 * m) * plane + (size_t)r * ldg + 2 * c;
+    double *go = ge + plane;
+    ge[0] = wn * (er * pr - ei * pi);
+    ge[1] = wn * (er * pi + ei * pr);
+    go[0] = wn * (orr * pr - oi * pi);
+    go[1] = wn * (orr * pi + oi * pr);
   }
 }
 
@@ -344,16 +521,91 @@ hipError_t sht_set_lds_limits(size_t bytes)
                              (int)bytes);
 }
 
-hipError_t launch_ring_dft(const void *maps, int fp64, int nmaps, int64_t npix, const ShtRing *rings, int R, int RP,
-                           int lmax, int ldg, double w, size_t lds, double *G, hipStream_t s)
+template <typename TM, int L>
+static void launch_ring_fft_t(const ShtLaunch &a, const TM *maps)
 {
-  dim3 grid(nmaps, R);
-  if (fp64)
-    hipLaunchKernelGGL(k_ring_dft<double>, grid, dim3(512), lds, s, (const double *)maps, npix, rings, R, RP, lmax,
-                       ldg, w, G);
-  else
-    hipLaunchKernelGGL(k_ring_dft<float>, grid, dim3(512), lds, s, (const float *)maps, npix, rings, R, RP, lmax,
-                       ldg, w, G);
+  using C = RingFftCfg<L>;
+  const int nlines = a.nmaps * a.nr;
+  hipLaunchKernelGGL((k_ring_fft<TM, L>), dim3((nlines + C::G - 1) / C::G), dim3(C::NT), 0, a.s, maps, a.npix, a.rings,
+                     a.R, a.RP, a.lmax, a.ldg, a.w, a.G, a.r0, a.nmaps, nlines, (const cx<double> *)a.tw);
+}
+
+template <typename TM, int M>
+static void launch_ring_blu_t(const ShtLaunch &a, const TM *maps)
+{
+  using C = BluCfg<M>;
+  const int nlines = a.nmaps * a.nr;
+  hipLaunchKernelGGL((k_ring_blu<TM, M>), dim3((nlines + C::G - 1) / C::G), dim3(C::NT), 0, a.s, maps, a.npix, a.rings,
+                     a.R, a.RP, a.lmax, a.ldg, a.w, a.G, a.r0, a.nmaps, nlines, a.woff, (const cx<double> *)a.wtab,
+                     a.boff, (const cx<double> *)a.B, (const cx<double> *)a.tw);
+}
+
+template <typename TM>
+static hipError_t launch_rings_t(const ShtLaunch &a, const TM *maps)
+{
+  switch (a.kind) {
+  case SHT_RING_SPLIT:
+    hipLaunchKernelGGL(k_ring_dft<TM>, dim3(a.nmaps, a.nr), dim3(512), a.lds, a.s, maps, a.npix, a.rings, a.R, a.RP,
+                       a.lmax, a.ldg, a.w, a.G, a.r0);
+    break;
+  case SHT_RING_FFT:
+    switch (a.len) {
+#define CRIME_FFT(LL) \
+  case LL: launch_ring_fft_t<TM, LL>(a, maps); break;
+      CRIME_FOR_EACH_RING_FFT_L(CRIME_FFT)
+#undef CRIME_FFT
+    default: return hipErrorInvalidValue;
+    }
+    break;
+  case SHT_RING_BLUESTEIN:
+    switch (a.len) {
+#define CRIME_BLU(MM) \
+  case MM: launch_ring_blu_t<TM, MM>(a, maps); break;
+      CRIME_FOR_EACH_BLU_M(CRIME_BLU)
+#undef CRIME_BLU
+    default: return hipErrorInvalidValue;
+    }
+    break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int ring_fft_length(int64_t nside)
+{
+  switch (4 * nside) {
+#define CRIME_FFT(LL) \
+  case LL: return LL;
+    CRIME_FOR_EACH_RING_FFT_L(CRIME_FFT)
+#undef CRIME_FFT
+  default: return 0;
+  }
+}
+
+hipError_t launch_rings(const ShtLaunch &a, const void *maps, int fp64)
+{
+  return fp64 ? launch_rings_t<double>(a, (const double *)maps) : launch_rings_t<float>(a, (const float *)maps);
+}
+
+template <int M>
+static void launch_blu_setup_t(const ShtRing *rings, int r0, int nr, const int64_t *woff, const void *wtab,
+                               const int64_t *boff, void *B, const void *tw, hipStream_t s)
+{
+  using C = BluCfg<M>;
+  hipLaunchKernelGGL((k_blu_setup<M>), dim3((nr + C::G - 1) / C::G), dim3(C::NT), 0, s, rings, r0, nr, woff,
+                     (const cx<double> *)wtab, boff, (cx<double> *)B, (const cx<double> *)tw);
+}
+
+hipError_t launch_blu_setup(int M, const ShtRing *rings, int r0, int nr, const int64_t *woff, const void *wtab,
+                            const int64_t *boff, void *B, const void *tw, hipStream_t s)
+{
+  switch (M) {
+#define CRIME_BLU(MM) \
+  case MM: launch_blu_setup_t<MM>(rings, r0, nr, woff, wtab, boff, B, tw, s); break;
+    CRIME_FOR_EACH_BLU_M(CRIME_BLU)
+#undef CRIME_BLU
+  default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

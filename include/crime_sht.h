@@ -48,9 +48,13 @@ int crime_sht_map2alm(crime_sht_t *p, int nmaps, const double *maps, double *alm
 int crime_sht_map2alm_device(crime_sht_t *p, int nmaps, const void *d_maps, int maps_fp64, double *alms);
 
 /* Angular power spectra of device maps.  cross = 1: every pair, cls[(a * nmaps + b) * (lmax + 1) + l];
- * cross = 0: autospectra only, cls[a * (lmax + 1) + l]. */
+ * cross = 0: autospectra only, cls[a * (lmax + 1) + l].  cls = NULL keeps them in the plan's device
+ * buffer (crime_sht_device_cls) without a copy to the host. */
 int crime_sht_anafast_device(crime_sht_t *p, int nmaps, const void *d_maps, int maps_fp64, int cross,
                              double *cls);
+
+/* Device pointer and byte count of the last spectra computed (valid until the next call). */
+int crime_sht_device_cls(const crime_sht_t *p, const double **dptr, size_t *bytes);
 
 /* C_l^{ab} for a list of map pairs only: pairs[2 i], pairs[2 i + 1] index the nmaps device maps;
  * cls[i * (lmax + 1) + l]. */

@@ -105,7 +105,7 @@ def test_sht_null_and_bad_plans():
     assert lib.crime_sht_create(16, 64, 1, 0, C.byref(h)) == -1      # lmax > 4 nside - 1
     assert b"lmax" in lib.crime_sht_last_error(None)
     assert lib.crime_sht_create(16, 47, 0, 0, C.byref(h)) == -1      # no maps
-    assert lib.crime_sht_create(1024, 3071, 1, 0, C.byref(h)) == -1  # ring spectra beyond the LDS
+    assert lib.crime_sht_create(2048, 6143, 1, 0, C.byref(h)) == -1  # ring spectra beyond the LDS
     assert b"LDS" in lib.crime_sht_last_error(None)
 
 

@@ -22,7 +22,8 @@ def rel_err(a, b):
 
 @pytest.mark.parametrize("nside,lmax,nmaps,max_maps", [(1, 3, 2, 2), (2, 7, 3, 5), (4, 11, 1, 1), (4, 15, 2, 3),
                                                        (8, 23, 3, 3), (16, 47, 2, 40), (32, 95, 2, 2),
-                                                       (64, 191, 2, 2)])
+                                                       (64, 191, 2, 2), (24, 71, 2, 2), (48, 143, 3, 3),
+                                                       (1024, 48, 1, 1)])
 def test_map2alm_vs_oracle(nside, lmax, nmaps, max_maps):
     rng = np.random.default_rng(nside * 100 + lmax)
     maps = rng.normal(size=(nmaps, 12 * nside * nside))
