@@ -15,6 +15,15 @@
 
 namespace crime {
 
+// Layout of the ring spectra G: blocks of 16 rings, then m, then parity (E, O), then the 16 rings of
+// the block, then [re, im] per map (ldg doubles per ring row).  A GEMM k-step (16 rings at one m and
+// parity) is one contiguous 16 x ldg block, and the m-sweep of one ring stays within
+// (lmax + 1) * 32 * ldg doubles instead of spanning the whole array.
+__device__ __forceinline__ size_t g_block(int rb, int m, int par, int lmax, int ldg)
+{
+  return (((size_t)rb * (lmax + 1) + m) * 2 + par) * 16 * (size_t)ldg;
+}
+
 // ---------------------------------------------------------------------------------------------
 // ring spectra: one workgroup per (map, northern ring); the mirror ring is transformed alongside.
 // DFT of length n = P Q (P a power of two): S[j1][k1] = sum_j2 f[P j2 + j1] w^{P j2 k1},
@@ -100,7 +109,6 @@ __global__ __launch_bounds__(512) void k_ring_dft(const TM *__restrict__ maps, i
   ring_spectrum(mp + ri.first_n, n, K, P, tw, f, S, Dn);
   const bool eq = (r == R - 1);
   if (!eq) ring_spectrum(mp + ri.first_s, n, K, P, tw, f, S, Ds);
-  const size_t plane = (size_t)RP * ldg;         // one (m, parity) slice of G
   for (int m = threadIdx.x; m <= lmax; m += blockDim.x) {
     const int k = m % n;
     double pr = 1, pi = 0;
@@ -113,8 +121,8 @@ __global__ __launch_bounds__(512) void k_ring_dft(const TM *__restrict__ maps, i
     double2 b = eq ? make_double2(0.0, 0.0) : Ds[k];
     double er = a.x + b.x, ei = a.y + b.y;
     double orr = a.x - b.x, oi = a.y - b.y;
-    double *ge = G + (size_t)(2 * m) * plane + (size_t)r * ldg + 2 * c;
-    double *go = ge + plane;
+    double *ge = G + g_block(r >> 4, m, 0, lmax, ldg) + (size_t)(r & 15) * ldg + 2 * c;
+    double *go = ge + 16 * (size_t)ldg;
     ge[0] = w * (er * pr - ei * pi);
     ge[1] = w * (er * pi + ei * pr);
     go[0] = w * (orr * pr - oi * pi);
@@ -130,6 +138,11 @@ __global__ __launch_bounds__(512) void k_ring_dft(const TM *__restrict__ maps, i
 // ---------------------------------------------------------------------------------------------
 constexpr int sht_imax(int a, int b) { return a > b ? a : b; }
 constexpr int sht_imin(int a, int b) { return a < b ? a : b; }
+
+// LDS slot of element e of a line: one pad slot per 16 elements, so the radix-16/8 first pass,
+// whose lanes write 16 elements apart, spreads over all banks instead of hitting one
+__device__ __forceinline__ int lpad(int e) { return e + (e >> 4); }
+constexpr int lpadded(int L) { return L + L / 16; }
 
 // threads per line: L / RING_TC_DIV, at least 64
 #ifndef CRIME_RING_TC_DIV
@@ -152,7 +165,7 @@ __global__ __launch_bounds__(RingFftCfg<L>::NT) void k_ring_fft(const TM *__rest
                                                                const cx<double> *__restrict__ tw)
 {
   using C = RingFftCfg<L>;
-  __shared__ cx<double> lds[C::G * L];
+  __shared__ cx<double> lds[C::G * lpadded(L)];
   const int g = threadIdx.x / C::TC, t = threadIdx.x % C::TC;
   const int line = xcd_swizzle(blockIdx.x, gridDim.x) * C::G + g;
   const bool live = line < nlines;            // padding lines still join every barrier
@@ -161,19 +174,18 @@ __global__ __launch_bounds__(RingFftCfg<L>::NT) void k_ring_fft(const TM *__rest
   const bool eq = (r == R - 1);
   const TM *fn = maps + (size_t)c * npix + ri.first_n;
   const TM *fs = maps + (size_t)c * npix + ri.first_s;
-  cx<double> *buf = lds + g * L;
+  cx<double> *buf = lds + g * lpadded(L);
   auto load = [&](int e) -> cx<double> {
     if (!live) return {0.0, 0.0};
     return {(double)fn[e], eq ? 0.0 : (double)fs[e]};
   };
-  auto store = [&](int e, cx<double> v) { buf[e] = v; };
-  auto slot = [&](int e) -> cx<double> & { return buf[e]; };
+  auto store = [&](int e, cx<double> v) { buf[lpad(e)] = v; };
+  auto slot = [&](int e) -> cx<double> & { return buf[lpad(e)]; };
   Stockham<double, L, C::TC>::run(t, load, store, slot, tw);
   __syncthreads();
   if (!live) return;
-  const size_t plane = (size_t)RP * ldg;
   for (int m = t; m <= lmax; m += C::TC) {
-    const cx<double> zk = buf[m], zm = buf[(L - m) & (L - 1)];
+    const cx<double> zk = buf[lpad(m)], zm = buf[lpad((L - m) & (L - 1))];
     // D_N = conj(A_N), D_S = conj(A_S)
     const double dnr = 0.5 * (zk.x + zm.x), dni = -0.5 * (zk.y - zm.y);
     const double dsr = 0.5 * (zk.y + zm.y), dsi = 0.5 * (zk.x - zm.x);
@@ -183,8 +195,8 @@ __global__ __launch_bounds__(RingFftCfg<L>::NT) void k_ring_fft(const TM *__rest
       sincospi(-(double)tt / (double)L, &pi, &pr);
     }
     const double er = dnr + dsr, ei = dni + dsi, orr = dnr - dsr, oi = dni - dsi;
-    double *ge = G + (size_t)(2 * m) * plane + (size_t)r * ldg + 2 * c;
-    double *go = ge + plane;
+    double *ge = G + g_block(r >> 4, m, 0, lmax, ldg) + (size_t)(r & 15) * ldg + 2 * c;
+    double *go = ge + 16 * (size_t)ldg;
     ge[0] = w * (er * pr - ei * pi);
     ge[1] = w * (er * pi + ei * pr);
     go[0] = w * (orr * pr - oi * pi);
@@ -216,14 +228,14 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_blu_setup(const ShtRing *__re
                                                             const cx<double> *__restrict__ tw)
 {
   using C = BluCfg<M>;
-  __shared__ cx<double> lds[C::G * M];
+  __shared__ cx<double> lds[C::G * lpadded(M)];
   const int g = threadIdx.x / C::TC, t = threadIdx.x % C::TC;
   const int line = xcd_swizzle(blockIdx.x, gridDim.x) * C::G + g;
   const bool live = line < nr;
   const int r = r0 + (live ? line : 0);
   const int n = rings[r].nphi;
   const cx<double> *w = wtab + woff[r];
-  cx<double> *buf = lds + g * M;
+  cx<double> *buf = lds + g * lpadded(M);
   auto load = [&](int e) -> cx<double> {
     if (!live) return {0.0, 0.0};
     const int m = e < n ? e : (e > M - n ? M - e : -1);
@@ -233,7 +245,7 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_blu_setup(const ShtRing *__re
   auto store = [&](int e, cx<double> v) {
     if (live) B[boff[r] + e] = {v.x / (double)M, v.y / (double)M};
   };
-  auto slot = [&](int e) -> cx<double> & { return buf[e]; };
+  auto slot = [&](int e) -> cx<double> & { return buf[lpad(e)]; };
   Stockham<double, M, C::TC>::run(t, load, store, slot, tw);
 }
 
@@ -248,7 +260,7 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_ring_blu(const TM *__restrict
                                                            const cx<double> *__restrict__ tw)
 {
   using C = BluCfg<M>;
-  __shared__ cx<double> lds[C::G * M];
+  __shared__ cx<double> lds[C::G * lpadded(M)];
   const int g = threadIdx.x / C::TC, t = threadIdx.x % C::TC;
   const int line = xcd_swizzle(blockIdx.x, gridDim.x) * C::G + g;
   const bool live = line < nlines;
@@ -260,27 +272,26 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_ring_blu(const TM *__restrict
   const TM *fs = maps + (size_t)c * npix + ri.first_s;
   const cx<double> *w = wtab + woff[r];
   const cx<double> *Bk = B + boff[r];
-  cx<double> *buf = lds + g * M;
+  cx<double> *buf = lds + g * lpadded(M);
   auto load1 = [&](int e) -> cx<double> {
     if (!live || e >= n) return {0.0, 0.0};
     const cx<double> z = {(double)fn[e], eq ? 0.0 : (double)fs[e]};
     return cmul(z, w[e]);
   };
-  auto store1 = [&](int e, cx<double> v) { buf[e] = v; };
-  auto slot = [&](int e) -> cx<double> & { return buf[e]; };
+  auto store1 = [&](int e, cx<double> v) { buf[lpad(e)] = v; };
+  auto slot = [&](int e) -> cx<double> & { return buf[lpad(e)]; };
   Stockham<double, M, C::TC>::run(t, load1, store1, slot, tw);
   __syncthreads();
-  auto load2 = [&](int e) -> cx<double> { return cconj(cmul(buf[e], Bk[e])); };
+  auto load2 = [&](int e) -> cx<double> { return cconj(cmul(buf[lpad(e)], Bk[e])); };
   auto store2 = [&](int e, cx<double> v) {
-    if (e < n) buf[e] = cmul(w[e], cconj(v));
+    if (e < n) buf[lpad(e)] = cmul(w[e], cconj(v));
   };
   Stockham<double, M, C::TC>::template run<true>(t, load2, store2, slot, tw);
   __syncthreads();
   if (!live) return;
-  const size_t plane = (size_t)RP * ldg;
   for (int m = t; m <= lmax; m += C::TC) {
     const int k = m % n, km = k == 0 ? 0 : n - k;
-    const cx<double> zk = buf[k], zm = buf[km];
+    const cx<double> zk = buf[lpad(k)], zm = buf[lpad(km)];
     const double dnr = 0.5 * (zk.x + zm.x), dni = -0.5 * (zk.y - zm.y);
     const double dsr = 0.5 * (zk.y + zm.y), dsi = 0.5 * (zk.x - zm.x);
     double pr = 1, pi = 0;
@@ -289,8 +300,8 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_ring_blu(const TM *__restrict
       sincospi(-(double)tt / (double)n, &pi, &pr);
     }
     const double er = dnr + dsr, ei = dni + dsi, orr = dnr - dsr, oi = dni - dsi;
-    double *ge = G + (size_t)(2 * m) * plane + (size_t)r * ldg + 2 * c;
-    double *go = ge + plane;
+    double *ge = G + g_block(r >> 4, m, 0, lmax, ldg) + (size_t)(r & 15) * ldg + 2 * c;
+    double *go = ge + 16 * (size_t)ldg;
     ge[0] = wn * (er * pr - ei * pi);
     ge[1] = wn * (er * pi + ei * pr);
     go[0] = wn * (orr * pr - oi * pi);
@@ -363,6 +374,7 @@ __global__ __launch_bounds__(256) void k_legendre(const double2 *__restrict__ co
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 constexpr int GT = 64, GK = 16;
+static_assert(GK == 16, "a GEMM k-step is one 16-ring block of G");
 
 __global__ __launch_bounds__(256) void k_alm_gemm(const double *__restrict__ lam, const double *__restrict__ G,
                                                   const int4 *__restrict__ tiles, int ntiles, int ncoltiles,
@@ -382,8 +394,7 @@ __global__ __launch_bounds__(256) void k_alm_gemm(const double *__restrict__ lam
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int row0 = rt * GT, col0 = ct * GT;
   const double *lrow = lam + (size_t)(off[m] - off[m0]) * RP;   // row (l - m) of this m
-  const size_t plane = (size_t)RP * ldg;
-  const double *g = G + (size_t)(2 * m + par) * plane;
+
   dbl4 acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0, 0, 0, 0};
@@ -399,7 +410,7 @@ __global__ __launch_bounds__(256) void k_alm_gemm(const double *__restrict__ lam
     } else {
       As[ar][ak] = As[ar][ak + 1] = As[ar][ak + 2] = As[ar][ak + 3] = 0.0;
     }
-    const double *bp = g + (size_t)(k0 + bk) * ldg + col0 + bc;
+    const double *bp = G + g_block(k0 >> 4, m, par, lmax, ldg) + (size_t)bk * ldg + col0 + bc;
     for (int e = 0; e < 4; e++) Bs[bk][bc + e] = (col0 + bc + e < ncols) ? bp[e] : 0.0;
     __syncthreads();
 #pragma unroll
@@ -427,40 +438,78 @@ __global__ __launch_bounds__(256) void k_alm_gemm(const double *__restrict__ lam
 }
 
 // ---------------------------------------------------------------------------------------------
-// C_l from alm[idx][2 t + (0 re | 1 im)]: cross (16 x 16 map pairs per workgroup) or autospectra.
+// C_l from alm[idx][2 t + (0 re | 1 im)]: cross spectra of all map pairs, autospectra, listed pairs.
 // ---------------------------------------------------------------------------------------------
+// Cross spectra on FP64 MFMA: for each l, C_l = X Y^T / (2l + 1) over k = 2m + (0 re | 1 im), with
+// X[a][k] = w_m alm_a, Y[b][k] = alm_b, w_0 = 1, w_m = 2.  One workgroup per (64 x 64 map tile with
+// ta <= tb, l); off-diagonal tiles also write the mirror block, so C^{ab} and C^{ba} are equal bit for
+// bit.  Consecutive blocks of one XCD take consecutive l of one tile, so the [a][b][l] stores of
+// neighbouring l meet in that XCD's L2.
 __global__ __launch_bounds__(256) void k_cls_cross(const double *__restrict__ alm, int lmax, int nmaps, int ldo,
-                                                   double *__restrict__ cls)
+                                                   int ntile, double *__restrict__ cls)
 {
-  const int l = blockIdx.z;
-  const int a0 = blockIdx.x * 16, b0 = blockIdx.y * 16;
-  const int ta = threadIdx.x & 15, tb = threadIdx.x >> 4;
-  __shared__ double2 sa[16][16], sb[16][16];   // [m in chunk][map]
-  double acc = 0;
-  for (int mc = 0; mc <= l; mc += 16) {
-    {
-      const int mm = threadIdx.x >> 4, t = threadIdx.x & 15, m = mc + mm;
-      double2 va = make_double2(0, 0), vb = make_double2(0, 0);
-      if (m <= l) {
-        const int64_t idx = m > 0 ? l + (int64_t)m * lmax - (int64_t)m * (m - 1) / 2 : l;
-        const double *row = alm + (size_t)idx * ldo;
-        const double wgt = m ? 2.0 : 1.0;
-        if (a0 + t < nmaps) va = make_double2(wgt * row[2 * (a0 + t)], wgt * row[2 * (a0 + t) + 1]);
-        if (b0 + t < nmaps) vb = make_double2(row[2 * (b0 + t)], row[2 * (b0 + t) + 1]);
-      }
-      sa[mm][t] = va;
-      sb[mm][t] = vb;
+  const int logical = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int nl = lmax + 1;
+  const int l = lmax - logical % nl;
+  int rem = logical / nl, ta = 0;
+  while (rem >= ntile - ta) {
+    rem -= ntile - ta;
+    ta++;
+  }
+  const int tb = ta + rem;
+  const int a0 = ta * GT, b0 = tb * GT;
+  __shared__ double As[GT][GK + 1];
+  __shared__ double Bs[GK][GT + 2];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  const int mi = tid >> 5, q = 2 * (tid & 31);      // loads: m chunk row mi, maps q and q + 1
+  dbl4 acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0, 0, 0, 0};
+  for (int mc = 0; mc <= l; mc += GK / 2) {
+    const int m = mc + mi;
+    double2 xa0 = make_double2(0, 0), xa1 = xa0, yb0 = xa0, yb1 = xa0;
+    if (m <= l) {
+      const int64_t idx = m > 0 ? l + (int64_t)m * lmax - (int64_t)m * (m - 1) / 2 : l;
+      const double *row = alm + (size_t)idx * ldo;
+      const double wgt = m ? 2.0 : 1.0;
+      if (a0 + q < nmaps) xa0 = *(const double2 *)(row + 2 * (a0 + q));
+      if (a0 + q + 1 < nmaps) xa1 = *(const double2 *)(row + 2 * (a0 + q + 1));
+      if (b0 + q < nmaps) yb0 = *(const double2 *)(row + 2 * (b0 + q));
+      if (b0 + q + 1 < nmaps) yb1 = *(const double2 *)(row + 2 * (b0 + q + 1));
+      xa0.x *= wgt; xa0.y *= wgt; xa1.x *= wgt; xa1.y *= wgt;
     }
+    As[q][2 * mi] = xa0.x;
+    As[q][2 * mi + 1] = xa0.y;
+    As[q + 1][2 * mi] = xa1.x;
+    As[q + 1][2 * mi + 1] = xa1.y;
+    Bs[2 * mi][q] = yb0.x;
+    Bs[2 * mi + 1][q] = yb0.y;
+    Bs[2 * mi][q + 1] = yb1.x;
+    Bs[2 * mi + 1][q + 1] = yb1.y;
     __syncthreads();
 #pragma unroll
-    for (int mm = 0; mm < 16; mm++) {
-      double2 x = sa[mm][ta], y = sb[mm][tb];
-      acc = fma(x.x, y.x, fma(x.y, y.y, acc));
+    for (int kk = 0; kk < GK; kk += 4) {
+      double af[2], bf[2];
+      for (int i = 0; i < 2; i++) af[i] = As[wr + 16 * i + (lane & 15)][kk + (lane >> 4)];
+      for (int j = 0; j < 2; j++) bf[j] = Bs[kk + (lane >> 4)][wc + 16 * j + (lane & 15)];
+      for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
-  if (a0 + ta < nmaps && b0 + tb < nmaps)
-    cls[((size_t)(a0 + ta) * nmaps + (b0 + tb)) * (lmax + 1) + l] = acc / (2.0 * l + 1.0);
+  const double inv = 1.0 / (2.0 * l + 1.0);
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int v = 0; v < 4; v++) {
+        const int a = a0 + wr + 16 * i + (lane >> 4) + 4 * v;
+        const int b = b0 + wc + 16 * j + (lane & 15);
+        if (a < nmaps && b < nmaps) {
+          const double c = acc[i][j][v] * inv;
+          cls[((size_t)a * nmaps + b) * nl + l] = c;
+          if (ta != tb) cls[((size_t)b * nmaps + a) * nl + l] = c;
+        }
+      }
 }
 
 __global__ __launch_bounds__(256) void k_cls_auto(const double *__restrict__ alm, int lmax, int nmaps, int ldo,
@@ -631,8 +680,10 @@ hipError_t launch_alm_gemm(const double *lam, const double *G, const int4 *tiles
 hipError_t launch_cls(const double *alm, int lmax, int nmaps, int ldo, int cross, double *cls, hipStream_t s)
 {
   if (cross) {
-    dim3 grid((nmaps + 15) / 16, (nmaps + 15) / 16, lmax + 1);
-    hipLaunchKernelGGL(k_cls_cross, grid, dim3(256), 0, s, alm, lmax, nmaps, ldo, cls);
+    const int nt = (nmaps + GT - 1) / GT;
+    const int64_t nblk = (int64_t)nt * (nt + 1) / 2 * (lmax + 1);
+    if (nblk > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cls_cross, dim3((unsigned)nblk), dim3(256), 0, s, alm, lmax, nmaps, ldo, nt, cls);
   } else {
     dim3 grid((nmaps + 255) / 256, lmax + 1);
     hipLaunchKernelGGL(k_cls_auto, grid, dim3(256), 0, s, alm, lmax, nmaps, ldo, cls);
