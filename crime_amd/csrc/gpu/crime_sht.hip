@@ -315,7 +315,7 @@ static int create_impl(crime_sht *p, int64_t nside, int lmax, int max_maps, int 
       const int L = lmax - m + 1;
       for (int par = 0; par < 2; par++) {
         const int nr = (L - par + 1) / 2;
-        for (int rt = 0; rt * 64 < nr; rt++) tiles.push_back(make_int4(m, par, rt, nr));
+        for (int rt = 0; rt * ALM_ROWS < nr; rt++) tiles.push_back(make_int4(m, par, rt, nr));
       }
       m++;
     }

@@ -50,6 +50,11 @@ hipError_t launch_blu_setup(int M, const ShtRing *rings, int r0, int nr, const i
 hipError_t launch_legendre(const double2 *coef, const int64_t *off, int m0, int m1, int lmax, const double *zs,
                            const double *sts, int R, int RP, double *lam, hipStream_t s);
 // tiles: (m, parity, row tile, rows of that parity)
+// rows (l values of one parity) per alm GEMM tile
+#ifndef CRIME_ALM_ROWS
+#define CRIME_ALM_ROWS 128
+#endif
+constexpr int ALM_ROWS = CRIME_ALM_ROWS;
 hipError_t launch_alm_gemm(const double *lam, const double *G, const int4 *tiles, int ntiles, const int64_t *off,
                            int m0, int lmax, int RP, int ldg, int ncols, double *alm, int ldo, hipStream_t s);
 hipError_t launch_cls(const double *alm, int lmax, int nmaps, int ldo, int cross, double *cls, hipStream_t s);

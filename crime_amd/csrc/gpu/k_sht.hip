@@ -375,13 +375,18 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 constexpr int GT = 64, GK = 16;
 static_assert(GK == 16, "a GEMM k-step is one 16-ring block of G");
+static_assert(ALM_ROWS == 64 || ALM_ROWS == 128, "alm tiles of 64 or 128 rows");
 
+// alm[l][col] = sum_r lambda_lm(r) G[r][col] per (m, parity): a workgroup computes ALM_ROWS x GT of
+// it, its 4 waves 2 x 2 blocks of (ALM_ROWS / 2) x 32 with 16x16x4 FP64 MFMAs.
 __global__ __launch_bounds__(256) void k_alm_gemm(const double *__restrict__ lam, const double *__restrict__ G,
                                                   const int4 *__restrict__ tiles, int ntiles, int ncoltiles,
                                                   int tiles_per_xcd, const int64_t *__restrict__ off, int m0,
                                                   int lmax, int RP, int ldg, int ncols, double *__restrict__ alm,
                                                   int ldo)
 {
+  constexpr int RW = ALM_ROWS / 2, NI = RW / 16;     // rows per wave, MFMA row blocks per wave
+  constexpr int AL = ALM_ROWS * GK / 256;            // A doubles loaded per thread
   // XCD-aware order: consecutive logical tiles (which share Legendre rows) stay on one XCD's L2
   const int b = blockIdx.x;
   const int logical = (b % 8) * tiles_per_xcd + b / 8;
@@ -389,42 +394,58 @@ __global__ __launch_bounds__(256) void k_alm_gemm(const double *__restrict__ lam
   const int4 tl = tiles[logical / ncoltiles];
   const int ct = logical % ncoltiles;
   const int m = tl.x, par = tl.y, rt = tl.z, nrows = tl.w;
-  __shared__ double As[GT][GK + 1];
+  __shared__ double As[ALM_ROWS][GK + 1];
   __shared__ double Bs[GK][GT + 2];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int row0 = rt * GT, col0 = ct * GT;
+  const int row0 = rt * ALM_ROWS, col0 = ct * GT;
   const double *lrow = lam + (size_t)(off[m] - off[m0]) * RP;   // row (l - m) of this m
 
-  dbl4 acc[2][2];
-  for (int i = 0; i < 2; i++)
+  dbl4 acc[NI][2];
+  for (int i = 0; i < NI; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0, 0, 0, 0};
-  const int ar = tid >> 2, ak = (tid & 3) * 4;       // A loads: row ar, 4 rings
-  const int bk = tid >> 4, bc = (tid & 15) * 4;      // B loads: ring bk, 4 columns
+  const int ar = tid / (GK / AL), ak = (tid % (GK / AL)) * AL;   // A loads: row ar, AL rings
+  const int bk = tid >> 4, bc = (tid & 15) * 4;                    // B loads: ring bk, 4 columns
+  // Operands out of range are read from an in-range address and zeroed by a select, so the loop
+  // has no divergent branches (they would make the compiler shuttle the accumulators between
+  // AGPRs and VGPRs every step).  ncols is even and col0 + bc a multiple of 4, so each double2
+  // of B is wholly in range or wholly out.
   const bool arow_ok = row0 + ar < nrows;
-  const double *ap = lrow + (size_t)(par + 2 * (row0 + ar)) * RP + ak;
-  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  const double *ap = lrow + (size_t)(par + 2 * (arow_ok ? row0 + ar : 0)) * RP + ak;
+  const bool b0_ok = col0 + bc < ncols, b1_ok = col0 + bc + 2 < ncols;
+  const int bcol0 = b0_ok ? col0 + bc : 0, bcol1 = b1_ok ? col0 + bc + 2 : 0;
+  const int wr = (wv >> 1) * RW, wc = (wv & 1) * 32;
+  // the next k-step's operands are fetched into registers while this one's MFMAs run
+  double2 ra[AL / 2], rb0, rb1;
+  auto fetch = [&](int k0) {
+    for (int e = 0; e < AL / 2; e++) ra[e] = *(const double2 *)(ap + k0 + 2 * e);
+    const double *bp = G + g_block(k0 >> 4, m, par, lmax, ldg) + (size_t)bk * ldg;
+    rb0 = *(const double2 *)(bp + bcol0);
+    rb1 = *(const double2 *)(bp + bcol1);
+  };
+  fetch(0);
   for (int k0 = 0; k0 < RP; k0 += GK) {
-    if (arow_ok) {
-      double2 v0 = *(const double2 *)(ap + k0), v1 = *(const double2 *)(ap + k0 + 2);
-      As[ar][ak] = v0.x; As[ar][ak + 1] = v0.y; As[ar][ak + 2] = v1.x; As[ar][ak + 3] = v1.y;
-    } else {
-      As[ar][ak] = As[ar][ak + 1] = As[ar][ak + 2] = As[ar][ak + 3] = 0.0;
+    for (int e = 0; e < AL / 2; e++) {
+      As[ar][ak + 2 * e] = arow_ok ? ra[e].x : 0.0;
+      As[ar][ak + 2 * e + 1] = arow_ok ? ra[e].y : 0.0;
     }
-    const double *bp = G + g_block(k0 >> 4, m, par, lmax, ldg) + (size_t)bk * ldg + col0 + bc;
-    for (int e = 0; e < 4; e++) Bs[bk][bc + e] = (col0 + bc + e < ncols) ? bp[e] : 0.0;
+    Bs[bk][bc] = b0_ok ? rb0.x : 0.0;
+    Bs[bk][bc + 1] = b0_ok ? rb0.y : 0.0;
+    Bs[bk][bc + 2] = b1_ok ? rb1.x : 0.0;
+    Bs[bk][bc + 3] = b1_ok ? rb1.y : 0.0;
     __syncthreads();
+    fetch(k0 + GK < RP ? k0 + GK : k0);   // the last step refetches its own operands: no branch
 #pragma unroll
     for (int kk = 0; kk < GK; kk += 4) {
-      double af[2], bf[2];
-      for (int i = 0; i < 2; i++) af[i] = As[wr + 16 * i + (lane & 15)][kk + (lane >> 4)];
+      double af[NI], bf[2];
+      for (int i = 0; i < NI; i++) af[i] = As[wr + 16 * i + (lane & 15)][kk + (lane >> 4)];
       for (int j = 0; j < 2; j++) bf[j] = Bs[kk + (lane >> 4)][wc + 16 * j + (lane & 15)];
-      for (int i = 0; i < 2; i++)
+      for (int i = 0; i < NI; i++)
         for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
   const int64_t base_idx = (int64_t)m * lmax - (int64_t)m * (m - 1) / 2;   // he_indexlm(l, m) - l
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < NI; i++)
     for (int j = 0; j < 2; j++)
       for (int v = 0; v < 4; v++) {
         const int row = row0 + wr + 16 * i + (lane >> 4) + 4 * v;
