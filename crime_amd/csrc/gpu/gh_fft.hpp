@@ -6,7 +6,7 @@
 // element (j/Ns)*Ns*R + (j mod Ns) + s*Ns (Stockham autosort, so no bit reversal is needed).
 // Pass 0 reads through a caller functor (global memory or an on-the-fly generator), intermediate
 // passes exchange through LDS, the last pass writes through a caller functor (epilogue).
-// Radix plan: as many 16s as possible, then 8/4/2, then 3s.  Inverse transforms (sign +).
+// Radix plan: as many 16s (or 8s) as possible, then 8/4/2, then 3s.  Inverse transforms (sign +).
 #pragma once
 
 #include "gh_device.hpp"
@@ -18,7 +18,7 @@ struct RadixPlan {
   int r[10];
 };
 
-constexpr RadixPlan make_plan(int L)
+constexpr RadixPlan make_plan(int L, int maxr = 16)
 {
   RadixPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int m = L, b = 0, a = 0;
@@ -29,6 +29,23 @@ constexpr RadixPlan make_plan(int L)
   while (m % 2 == 0) {
     m /= 2;
     a++;
+  }
+  if (maxr == 8) {   // radix 8 at most: twice the threads per line of the radix-16 plan, one pass more
+    int n8 = a / 3, rem = a % 3;
+    if (rem == 1 && n8 >= 1) n8 -= 1;
+    for (int i = 0; i < n8; i++) p.r[p.n++] = 8;
+    if (rem == 1) {
+      if (a >= 4) {
+        p.r[p.n++] = 4;
+        p.r[p.n++] = 4;
+      }
+      else {
+        p.r[p.n++] = 2;
+      }
+    }
+    if (rem == 2) p.r[p.n++] = 4;
+    for (int i = 0; i < b; i++) p.r[p.n++] = 3;
+    return p;
   }
   int n16 = a / 4, rem = a % 4;
   if (rem == 1 && n16 >= 1) n16 -= 1;
@@ -52,9 +69,9 @@ constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
 
 // largest number of threads per line such that every pass has a whole number of butterflies per
 // thread, capped at tcmax
-constexpr int plan_tc(int L, int tcmax)
+constexpr int plan_tc(int L, int tcmax, int maxr = 16)
 {
-  RadixPlan p = make_plan(L);
+  RadixPlan p = make_plan(L, maxr);
   int g = L / p.r[0];
   for (int i = 1; i < p.n; i++) g = cgcd(g, L / p.r[i]);
   while (g > tcmax) g = (g % 2 == 0) ? g / 2 : g / 3;
@@ -173,9 +190,9 @@ __device__ __forceinline__ void dft(cx<T> *v)
 // ---------------------------------------------------------------------------------------------
 // the engine
 // ---------------------------------------------------------------------------------------------
-template <typename T, int L, int TC>
+template <typename T, int L, int TC, int MAXR = 16>
 struct Stockham {
-  static constexpr RadixPlan plan = make_plan(L);
+  static constexpr RadixPlan plan = make_plan(L, MAXR);
   static constexpr int NP = plan.n;
   static constexpr int E = L / TC;
 

@@ -144,14 +144,18 @@ constexpr int sht_imin(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int lpad(int e) { return e + (e >> 4); }
 constexpr int lpadded(int L) { return L + L / 16; }
 
-// threads per line: L / RING_TC_DIV, at least 64
+// threads per line: L / RING_TC_DIV, at least 64; largest radix of the ring transforms
 #ifndef CRIME_RING_TC_DIV
 #define CRIME_RING_TC_DIV 16
+#endif
+#ifndef CRIME_RING_MAXR
+#define CRIME_RING_MAXR 16
 #endif
 
 template <int L>
 struct RingFftCfg {
-  static constexpr int TC = plan_tc(L, sht_imax(64, L / CRIME_RING_TC_DIV));
+  static constexpr int MAXR = CRIME_RING_MAXR;
+  static constexpr int TC = plan_tc(L, sht_imax(64, L / CRIME_RING_TC_DIV), MAXR);
   // lines per workgroup: at least 128 threads where 64 KB of LDS allows
   static constexpr int G = sht_imax(1, sht_imin(sht_imax(4, 128 / TC), 65536 / (16 * L)));
   static constexpr int NT = G * TC;
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(RingFftCfg<L>::NT) void k_ring_fft(const TM *__rest
   };
   auto store = [&](int e, cx<double> v) { buf[lpad(e)] = v; };
   auto slot = [&](int e) -> cx<double> & { return buf[lpad(e)]; };
-  Stockham<double, L, C::TC>::run(t, load, store, slot, tw);
+  Stockham<double, L, C::TC, C::MAXR>::run(t, load, store, slot, tw);
   __syncthreads();
   if (!live) return;
   for (int m = t; m <= lmax; m += C::TC) {
@@ -214,7 +218,8 @@ __global__ __launch_bounds__(RingFftCfg<L>::NT) void k_ring_fft(const TM *__rest
 // ---------------------------------------------------------------------------------------------
 template <int M>
 struct BluCfg {
-  static constexpr int TC = plan_tc(M, sht_imax(64, M / CRIME_RING_TC_DIV));
+  static constexpr int MAXR = CRIME_RING_MAXR;
+  static constexpr int TC = plan_tc(M, sht_imax(64, M / CRIME_RING_TC_DIV), MAXR);
   static constexpr int G = sht_imax(1, sht_imin(sht_imax(4, 128 / TC), 65536 / (16 * M)));
   static constexpr int NT = G * TC;
 };
@@ -246,7 +251,7 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_blu_setup(const ShtRing *__re
     if (live) B[boff[r] + e] = {v.x / (double)M, v.y / (double)M};
   };
   auto slot = [&](int e) -> cx<double> & { return buf[lpad(e)]; };
-  Stockham<double, M, C::TC>::run(t, load, store, slot, tw);
+  Stockham<double, M, C::TC, C::MAXR>::run(t, load, store, slot, tw);
 }
 
 template <typename TM, int M>
@@ -280,13 +285,13 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_ring_blu(const TM *__restrict
   };
   auto store1 = [&](int e, cx<double> v) { buf[lpad(e)] = v; };
   auto slot = [&](int e) -> cx<double> & { return buf[lpad(e)]; };
-  Stockham<double, M, C::TC>::run(t, load1, store1, slot, tw);
+  Stockham<double, M, C::TC, C::MAXR>::run(t, load1, store1, slot, tw);
   __syncthreads();
   auto load2 = [&](int e) -> cx<double> { return cconj(cmul(buf[lpad(e)], Bk[e])); };
   auto store2 = [&](int e, cx<double> v) {
     if (e < n) buf[lpad(e)] = cmul(w[e], cconj(v));
   };
-  Stockham<double, M, C::TC>::template run<true>(t, load2, store2, slot, tw);
+  Stockham<double, M, C::TC, C::MAXR>::template run<true>(t, load2, store2, slot, tw);
   __syncthreads();
   if (!live) return;
   for (int m = t; m <= lmax; m += C::TC) {
