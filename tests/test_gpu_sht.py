@@ -2,9 +2,9 @@
 (oracle/sht_oracle.c) on the same maps, and closed forms at the production Nside.
 
 Bars: alms to 1e-11 of the largest |a_lm| (FP64 throughout; the GPU and the oracle differ in summation
-order, ring transform and Legendre recursion); C_l to 1e-10 relative; at Nside 512 (lmax 1535) the
-closed forms of a_00, a_10, a_11 and ring-by-ring direct sums at sampled (l, m) to 1e-10 of
-max(|a_lm|, white-noise rms).
+order, ring transform and Legendre recursion); C_l to 1e-10 relative; at Nside 512 and 1024
+(lmax 3 Nside - 1) the closed forms of a_00, a_10, a_11 and ring-by-ring direct sums at sampled (l, m)
+to 1e-10 of max(|a_lm|, white-noise rms).
 """
 import numpy as np
 import pytest
@@ -86,15 +86,20 @@ def _alm_direct(maps, nside, l, m):
     return out * (4 * np.pi / npix)
 
 
-def test_production_nside512_closed_forms():
-    nside, lmax = 512, 1535
+@pytest.mark.parametrize("nside,lms", [
+    (512, [(2, 1), (40, 17), (700, 3), (1000, 999), (1535, 400), (1535, 1535)]),
+    # Nside 1024 (the largest mock's maps): 8192-point Bluestein convolutions for the polar rings,
+    # the 4096-point equatorial FFT and a Legendre table rebuilt in m-chunks
+    (1024, [(3, 2), (1500, 700), (2900, 2048), (3071, 17), (3071, 3071)])])
+def test_production_closed_forms(nside, lms):
+    lmax = 3 * nside - 1
     npix = 12 * nside * nside
     rng = np.random.default_rng(11)
     maps = rng.normal(size=(2, npix)) + np.array([[2.0], [0.0]])
     with crime_amd.Sht(nside, lmax, max_maps=2) as sh:
         alm = sh.map2alm(maps)
         st = sh.stats()
-    print(f"Nside 512 map2alm of 2 maps: {st}")
+    print(f"Nside {nside} map2alm of 2 maps: {st}")
     idx = crime_amd.alm_index
     w = 4 * np.pi / npix
     # textbook Y_00, Y_10, Y_11 at the pixel centres
@@ -111,7 +116,7 @@ def test_production_nside512_closed_forms():
     noise = 4 * np.pi / np.sqrt(npix)   # ~ rms |a_lm| of a unit white-noise map
     for (l, m), ref in closed.items():
         assert np.abs(alm[:, idx(l, m, lmax)] - ref).max() < 1e-10 * max(np.abs(ref).max(), noise), (l, m)
-    for l, m in [(2, 1), (40, 17), (700, 3), (1000, 999), (1535, 400), (1535, 1535)]:
+    for l, m in lms:
         ref = _alm_direct(maps, nside, l, m)
         assert np.abs(alm[:, idx(l, m, lmax)] - ref).max() < 1e-10 * max(np.abs(ref).max(), noise), (l, m)
 
