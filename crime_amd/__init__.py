@@ -20,7 +20,7 @@ from . import _ffi
 from ._ffi import (BIAS_CONST, BIAS_LINEAR, BIAS_REFERENCE, FP32, FP64, GRID_DENS, GRID_DZRSD, GRID_MASS,
                    GRID_RVEL, GRID_VPOT, MODES_INJECTED, MODES_PHILOX, NU_TABLE, NU_UNIFORM, CrimeGhDesc)
 
-__all__ = ["Params", "GetHI", "Sht", "Batch", "run_batch", "alm_index", "CrimeError", "write_ini",
+__all__ = ["Params", "GetHI", "Sht", "Batch", "run_batch", "alm_index", "jt_instrument", "CrimeError", "write_ini",
            "write_bbks_pk", "FP32", "FP64",
            "GRID_DENS", "GRID_VPOT", "GRID_RVEL", "GRID_MASS", "GRID_DZRSD", "MODES_PHILOX",
            "MODES_INJECTED", "NU_TABLE", "NU_UNIFORM", "BIAS_REFERENCE", "BIAS_CONST", "BIAS_LINEAR"]
@@ -278,9 +278,23 @@ def alm_index(l, m, lmax):
     return l + m * lmax - m * (m - 1) // 2 if m > 0 else l
 
 
+def jt_instrument(nu0, nuf, nside, dish_diameter=0.0, tsys=0.0, time_total=1.0, n_dish=1, fsky=1.0):
+    """Per-channel beam FWHM (arcmin) and noise sigma2 (mK^2) of JoinT (src/main_jt.c:27-67, 270-311)
+    for channels [nu0, nuf) in MHz: nu = (nu0 + nuf)/2 (src/io_jt.c:231), fwhm = 1030610.55/(nu D)
+    (0 when D <= 0), sigma2 = fsky Npix tsys^2 / (3.6e9 dnu t_tot n_dish) (0 when tsys <= 0), tsys in mK,
+    t_tot in hours."""
+    nu0, nuf = np.asarray(nu0, np.float64), np.asarray(nuf, np.float64)
+    nu, dnu = 0.5 * (nu0 + nuf), nuf - nu0
+    fwhm = 1030610.55 / (nu * dish_diameter) if dish_diameter > 0 else np.zeros_like(nu)
+    npix = 12 * int(nside) ** 2
+    sig2 = (fsky * npix * tsys * tsys / (3.6e9 * dnu * time_total * n_dish) if tsys > 0 else np.zeros_like(nu))
+    return fwhm, sig2
+
+
 class Sht:
-    """HEALPix analysis plan of libcrime_gh.so (include/crime_sht.h): map2alm as he_map2alm
-    (src/healpix_extra.c:88-130) does it, and angular power spectra, in FP64 on the device."""
+    """HEALPix plan of libcrime_gh.so (include/crime_sht.h): map2alm and alm2map as he_map2alm and
+    he_alm2map (src/healpix_extra.c:44-130) do them, angular power spectra, and JoinT's beam, noise and
+    mask stages (src/main_jt.c:211-311), in FP64 on the device."""
 
     def __init__(self, nside, lmax=None, max_maps=1, device=0):
         self._lib = _ffi.gpu_lib()
@@ -352,6 +366,50 @@ class Sht:
         if pairs is not None:
             return self.cls_pairs_device(p, n, gh.precision == FP64, pairs)
         return self.anafast_device(p, n, gh.precision == FP64, cross, to_host)
+
+    def alm2map(self, alms):
+        """alms (nmaps, nalm) complex, he_indexlm order, on the host -> real maps (nmaps, Npix) RING, as
+        he_alm2map (src/healpix_extra.c:44-86) computes them."""
+        a = np.ascontiguousarray(np.atleast_2d(alms), dtype=np.complex128)
+        assert a.shape[1] == self.nalm
+        out = np.empty((a.shape[0], self.npix))
+        self._check(self._lib.crime_sht_alm2map(self._p, a.shape[0], a.ctypes.data_as(_ffi._dp), _dptr(out)))
+        return out
+
+    def alm2map_device(self, d_alms, nmaps, d_maps, fp64):
+        """Device alms (nmaps x nalm complex) into device maps (FP32 or FP64)."""
+        self._check(self._lib.crime_sht_alm2map_device(self._p, nmaps, C.c_void_p(d_alms), C.c_void_p(d_maps),
+                                                       int(fp64)))
+
+    def smooth_device(self, dptr, nmaps, fp64, fwhm_arcmin):
+        """JoinT's apply_beam in place: map2alm, Gaussian beam of fwhm_arcmin[t] per map, alm2map."""
+        f = np.ascontiguousarray(np.broadcast_to(np.asarray(fwhm_arcmin, np.float64), (nmaps,)))
+        self._check(self._lib.crime_sht_smooth_device(self._p, nmaps, C.c_void_p(dptr), int(fp64), _dptr(f)))
+
+    def add_noise_device(self, dptr, nmaps, fp64, sigma2, seed):
+        """JoinT's add_noise: += sqrt(-sigma2[t] ln(1-u2)) cos(2 pi u1) per pixel (variance sigma2 / 2)."""
+        s2 = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma2, np.float64), (nmaps,)))
+        self._check(self._lib.crime_sht_add_noise_device(self._p, nmaps, C.c_void_p(dptr), int(fp64), _dptr(s2),
+                                                         int(seed) & 0xFFFFFFFF))
+
+    def mask_device(self, dptr, nmaps, fp64, mask):
+        """JoinT's apply_mask: zero every pixel where mask (Npix, nonzero = observed) is 0."""
+        mk = np.ascontiguousarray(np.asarray(mask) != 0, dtype=np.uint8)
+        assert mk.shape == (self.npix,)
+        self._check(self._lib.crime_sht_mask_device(self._p, nmaps, C.c_void_p(dptr), int(fp64),
+                                                    mk.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def instrument_of(self, gh, fwhm_arcmin=None, sigma2=None, seed=0, mask=None):
+        """The JoinT stages in its order (src/main_jt.c:362-382) on a GetHI handle's resident maps: beam if
+        fwhm_arcmin is given, noise if sigma2 is given, then the mask if one is given."""
+        p, _ = gh.device_maps()
+        fp64 = gh.precision == FP64
+        if fwhm_arcmin is not None:
+            self.smooth_device(p, gh.n_nu, fp64, fwhm_arcmin)
+        if sigma2 is not None:
+            self.add_noise_device(p, gh.n_nu, fp64, sigma2, seed)
+        if mask is not None:
+            self.mask_device(p, gh.n_nu, fp64, mask)
 
     def stats(self):
         st = _ffi.CrimeShtStats()

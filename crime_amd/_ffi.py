@@ -186,6 +186,16 @@ def gpu_lib():
         lib.crime_sht_anafast.argtypes = [H, C.c_int, _dp, C.c_int, _dp]
         lib.crime_sht_device_cls.restype = C.c_int
         lib.crime_sht_device_cls.argtypes = [H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
+        lib.crime_sht_alm2map.restype = C.c_int
+        lib.crime_sht_alm2map.argtypes = [H, C.c_int, _dp, _dp]
+        lib.crime_sht_alm2map_device.restype = C.c_int
+        lib.crime_sht_alm2map_device.argtypes = [H, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        lib.crime_sht_smooth_device.restype = C.c_int
+        lib.crime_sht_smooth_device.argtypes = [H, C.c_int, C.c_void_p, C.c_int, _dp]
+        lib.crime_sht_add_noise_device.restype = C.c_int
+        lib.crime_sht_add_noise_device.argtypes = [H, C.c_int, C.c_void_p, C.c_int, _dp, C.c_uint32]
+        lib.crime_sht_mask_device.restype = C.c_int
+        lib.crime_sht_mask_device.argtypes = [H, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_uint8)]
         lib.crime_sht_get_stats.restype = C.c_int
         lib.crime_sht_get_stats.argtypes = [H, C.POINTER(CrimeShtStats)]
         lib.crime_mlp_param_count.restype = C.c_int64

@@ -6,6 +6,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -25,7 +26,8 @@ constexpr int BLU_MAX_M = 8192;   // largest compiled Bluestein convolution
 
 struct Chunk {
   int m0, m1;        // [m0, m1)
-  int tile0, ntile;  // slice of the tile list
+  int tile0, ntile;  // slice of the analysis tile list
+  int syn0, nsyn;    // slice of the synthesis tile list
 };
 
 }  // namespace
@@ -41,7 +43,10 @@ struct crime_sht {
   double *d_z = nullptr, *d_st = nullptr;
   double2 *d_coef = nullptr;
   int64_t *d_off = nullptr;
-  int4 *d_tiles = nullptr;
+  int4 *d_tiles = nullptr, *d_syn_tiles = nullptr;
+  double *d_sig2 = nullptr;     // per-map beam sigma^2 or noise variance
+  uint8_t *d_mask = nullptr;    // Npix, allocated by the first mask call
+  bool syn_ok = true;           // every split-DFT ring is shorter than 32 pixels
   double *d_G = nullptr, *d_alm = nullptr, *d_lam = nullptr;
   double *d_cls = nullptr;
   size_t cls_cap = 0, cls_bytes = 0;
@@ -88,7 +93,8 @@ extern "C" void crime_sht_destroy(crime_sht_t *p)
   if (!p) return;
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
-  void *bufs[] = {p->d_rings, p->d_z, p->d_st, p->d_coef, p->d_off, p->d_tiles, p->d_G,
+  void *bufs[] = {p->d_rings, p->d_z, p->d_st, p->d_coef, p->d_off, p->d_tiles, p->d_syn_tiles, p->d_sig2,
+                  p->d_mask, p->d_G,
                   p->d_alm, p->d_lam, p->d_cls, p->d_unpack, p->d_hmaps, p->d_tw,
                   p->d_woff, p->d_boff, p->d_wtab, p->d_btab};
   for (void *b : bufs)
@@ -182,6 +188,7 @@ static hipError_t plan_rings(crime_sht *p, const std::vector<ShtRing> &rings)
   for (int r = 0; r < R;) {
     int e = r + 1;
     while (e < R && kind[e] == kind[r] && len[e] == len[r]) e++;
+    if (kind[r] == SHT_RING_SPLIT && rings[r].nphi > 32) p->syn_ok = false;
     ShtLaunch g{};
     g.kind = kind[r];
     g.len = len[r];
@@ -303,12 +310,14 @@ static int create_impl(crime_sht *p, int64_t nside, int lmax, int max_maps, int 
 
   // m chunks of the Legendre table and the GEMM tiles of each chunk
   const size_t row_bytes = (size_t)p->RP * sizeof(double);
-  std::vector<int4> tiles;
+  std::vector<int4> tiles, syn_tiles;
   size_t max_chunk_rows = 0;
+  const int nringtiles = (p->R + ALM_ROWS - 1) / ALM_ROWS;
   for (int m = 0; m <= lmax;) {
     Chunk c;
     c.m0 = m;
     c.tile0 = (int)tiles.size();
+    c.syn0 = (int)syn_tiles.size();
     size_t rows = 0;
     while (m <= lmax && (rows == 0 || (rows + (size_t)(lmax - m + 1)) * row_bytes <= LAMBDA_BUDGET)) {
       rows += (size_t)(lmax - m + 1);
@@ -316,23 +325,29 @@ static int create_impl(crime_sht *p, int64_t nside, int lmax, int max_maps, int 
       for (int par = 0; par < 2; par++) {
         const int nr = (L - par + 1) / 2;
         for (int rt = 0; rt * ALM_ROWS < nr; rt++) tiles.push_back(make_int4(m, par, rt, nr));
+        // synthesis: every ring tile, also when this parity has no rows (its spectra are zero)
+        for (int rt = 0; rt < nringtiles; rt++) syn_tiles.push_back(make_int4(m, par, rt, nr));
       }
       m++;
     }
     c.m1 = m;
     c.ntile = (int)tiles.size() - c.tile0;
+    c.nsyn = (int)syn_tiles.size() - c.syn0;
     if (rows > max_chunk_rows) max_chunk_rows = rows;
     p->chunks.push_back(c);
   }
   SHT_HIP(p, hipMalloc((void **)&p->d_tiles, tiles.size() * sizeof(int4)));
   SHT_HIP(p, hipMemcpy(p->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  SHT_HIP(p, hipMalloc((void **)&p->d_syn_tiles, syn_tiles.size() * sizeof(int4)));
+  SHT_HIP(p, hipMemcpy(p->d_syn_tiles, syn_tiles.data(), syn_tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  SHT_HIP(p, hipMalloc((void **)&p->d_sig2, (size_t)max_maps * sizeof(double)));
   SHT_HIP(p, hipMalloc((void **)&p->d_lam, max_chunk_rows * row_bytes));
 
   const size_t gsz = (size_t)(lmax + 1) * 2 * p->RP * p->ldg * sizeof(double);
   SHT_HIP(p, hipMalloc((void **)&p->d_G, gsz));
   SHT_HIP(p, hipMemset(p->d_G, 0, gsz));   // rows of the padding rings stay zero
   SHT_HIP(p, hipMalloc((void **)&p->d_alm, (size_t)p->nalm * p->ldo * sizeof(double)));
-  p->ev.resize(2 * p->chunks.size() + 4);
+  p->ev.resize(4 * p->chunks.size() + 8);   // analysis events, then synthesis events
   for (auto &e : p->ev) SHT_HIP(p, hipEventCreate(&e));
 
   p->resident = p->chunks.size() == 1;
@@ -368,6 +383,23 @@ static float elapsed(hipEvent_t a, hipEvent_t b)
 }
 
 // ring spectra + alm GEMMs of nmaps device maps into p->d_alm; stage times into p->st
+// a ring group's launch record for nmaps maps
+static ShtLaunch ring_launch(const crime_sht *p, ShtLaunch g, int nmaps)
+{
+  g.nmaps = nmaps;
+  g.npix = p->npix;
+  g.rings = p->d_rings;
+  g.R = p->R;
+  g.RP = p->RP;
+  g.lmax = p->lmax;
+  g.ldg = p->ldg;
+  g.w = 4.0 * M_PI / (double)p->npix;
+  g.lds = p->lds;
+  g.G = p->d_G;
+  g.s = p->stream;
+  return g;
+}
+
 static int analyse(crime_sht *p, int nmaps, const void *d_maps, int fp64)
 {
   if (nmaps < 1 || nmaps > p->max_maps)
@@ -375,20 +407,7 @@ static int analyse(crime_sht *p, int nmaps, const void *d_maps, int fp64)
   if (!d_maps) return sht_error(p, CRIME_GH_EINVAL, "maps are NULL");
   hipEvent_t *ev = p->ev.data();
   SHT_HIP(p, hipEventRecord(ev[0], p->stream));
-  for (ShtLaunch g : p->groups) {
-    g.nmaps = nmaps;
-    g.npix = p->npix;
-    g.rings = p->d_rings;
-    g.R = p->R;
-    g.RP = p->RP;
-    g.lmax = p->lmax;
-    g.ldg = p->ldg;
-    g.w = 4.0 * M_PI / (double)p->npix;
-    g.lds = p->lds;
-    g.G = p->d_G;
-    g.s = p->stream;
-    SHT_HIP(p, launch_rings(g, d_maps, fp64));
-  }
+  for (const ShtLaunch &g : p->groups) SHT_HIP(p, launch_rings(ring_launch(p, g, nmaps), d_maps, fp64));
   SHT_HIP(p, hipEventRecord(ev[1], p->stream));
   for (size_t i = 0; i < p->chunks.size(); i++) {
     const Chunk &c = p->chunks[i];
@@ -531,6 +550,155 @@ extern "C" int crime_sht_anafast(crime_sht_t *p, int nmaps, const double *maps, 
   int rc = upload_maps(p, nmaps, maps);
   if (rc) return rc;
   return crime_sht_anafast_device(p, nmaps, p->d_hmaps, 1, cross, cls);
+}
+
+// ---------------------------------------------------------------------------------------------
+// synthesis: alm2map, the JoinT-like beam, noise and mask
+// ---------------------------------------------------------------------------------------------
+static hipEvent_t *syn_events(crime_sht *p) { return p->ev.data() + 2 * p->chunks.size() + 4; }
+
+// p->d_alm (nmaps maps) -> d_out (FP32 or FP64 RING maps): per m-chunk Legendre table (unless
+// resident) and the transposed GEMM into G, then the ring syntheses
+static int synthesize(crime_sht *p, int nmaps, void *d_out, int fp64)
+{
+  hipEvent_t *ev = syn_events(p);
+  SHT_HIP(p, hipEventRecord(ev[0], p->stream));
+  for (size_t i = 0; i < p->chunks.size(); i++) {
+    const Chunk &c = p->chunks[i];
+    if (!p->resident)
+      SHT_HIP(p, launch_legendre(p->d_coef, p->d_off, c.m0, c.m1, p->lmax, p->d_z, p->d_st, p->R, p->RP, p->d_lam,
+                                 p->stream));
+    SHT_HIP(p, hipEventRecord(ev[1 + 2 * i], p->stream));
+    SHT_HIP(p, launch_syn_gemm(p->d_lam, p->d_alm, p->d_G, p->d_syn_tiles + c.syn0, c.nsyn, p->d_off, c.m0, p->lmax,
+                               p->R, p->RP, p->ldg, 2 * nmaps, p->ldo, p->stream));
+    SHT_HIP(p, hipEventRecord(ev[2 + 2 * i], p->stream));
+  }
+  for (const ShtLaunch &g : p->groups) SHT_HIP(p, launch_syn_rings(ring_launch(p, g, nmaps), d_out, fp64));
+  SHT_HIP(p, hipEventRecord(ev[1 + 2 * p->chunks.size()], p->stream));
+  return 0;
+}
+
+// after a synthesis: wait, and add its stage times to p->st (cleared first unless `accumulate`)
+static int finish_syn(crime_sht *p, bool accumulate)
+{
+  SHT_HIP(p, hipStreamSynchronize(p->stream));
+  hipEvent_t *ev = syn_events(p);
+  const size_t nc = p->chunks.size();
+  if (!accumulate) p->st = crime_sht_stats{};
+  hipEvent_t prev = ev[0];
+  for (size_t i = 0; i < nc; i++) {
+    p->st.ms_legendre += elapsed(prev, ev[1 + 2 * i]);
+    p->st.ms_alm += elapsed(ev[1 + 2 * i], ev[2 + 2 * i]);
+    prev = ev[2 + 2 * i];
+  }
+  p->st.ms_rings += elapsed(ev[2 * nc], ev[1 + 2 * nc]);
+  if (!accumulate) p->st.ms_total = elapsed(ev[0], ev[1 + 2 * nc]);
+  return 0;
+}
+
+static int syn_check(crime_sht *p, int nmaps)
+{
+  if (!p->syn_ok) return sht_error(p, CRIME_GH_EINVAL, "nside %lld: no synthesis plan", (long long)p->nside);
+  if (nmaps < 1 || nmaps > p->max_maps)
+    return sht_error(p, CRIME_GH_EINVAL, "nmaps %d outside [1, %d]", nmaps, p->max_maps);
+  return 0;
+}
+
+extern "C" int crime_sht_alm2map_device(crime_sht_t *p, int nmaps, const double *d_alms, void *d_maps, int maps_fp64)
+{
+  if (!p) return sht_error(nullptr, CRIME_GH_EINVAL, "plan is NULL");
+  int rc = syn_check(p, nmaps);
+  if (rc) return rc;
+  if (!d_alms || !d_maps) return sht_error(p, CRIME_GH_EINVAL, "alms or maps are NULL");
+  SHT_HIP(p, hipSetDevice(p->device));
+  SHT_HIP(p, launch_alm_pack(d_alms, p->nalm, nmaps, p->ldo, p->d_alm, p->stream));
+  rc = synthesize(p, nmaps, d_maps, maps_fp64);
+  if (rc) return rc;
+  return finish_syn(p, false);
+}
+
+extern "C" int crime_sht_alm2map(crime_sht_t *p, int nmaps, const double *alms, double *maps)
+{
+  if (!p) return sht_error(nullptr, CRIME_GH_EINVAL, "plan is NULL");
+  int rc = syn_check(p, nmaps);
+  if (rc) return rc;
+  if (!alms || !maps) return sht_error(p, CRIME_GH_EINVAL, "alms or maps are NULL");
+  SHT_HIP(p, hipSetDevice(p->device));
+  if (!p->d_unpack)
+    SHT_HIP(p, hipMalloc((void **)&p->d_unpack, (size_t)p->nalm * p->max_maps * 2 * sizeof(double)));
+  if (!p->d_hmaps) SHT_HIP(p, hipMalloc((void **)&p->d_hmaps, (size_t)p->npix * p->max_maps * sizeof(double)));
+  SHT_HIP(p, hipMemcpyAsync(p->d_unpack, alms, (size_t)p->nalm * nmaps * 2 * sizeof(double), hipMemcpyHostToDevice,
+                            p->stream));
+  rc = crime_sht_alm2map_device(p, nmaps, p->d_unpack, p->d_hmaps, 1);
+  if (rc) return rc;
+  SHT_HIP(p, hipMemcpy(maps, p->d_hmaps, (size_t)p->npix * nmaps * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// FWHM2SIGMA of he_generate_beam_window (src/healpix_extra.c:389): arcmin -> radians / sqrt(8 ln 2)
+static const double SHT_FWHM2SIGMA = 0.00012352884853326381;
+
+extern "C" int crime_sht_smooth_device(crime_sht_t *p, int nmaps, void *d_maps, int maps_fp64,
+                                       const double *fwhm_arcmin)
+{
+  if (!p) return sht_error(nullptr, CRIME_GH_EINVAL, "plan is NULL");
+  int rc = syn_check(p, nmaps);
+  if (rc) return rc;
+  if (!fwhm_arcmin) return sht_error(p, CRIME_GH_EINVAL, "fwhm is NULL");
+  std::vector<double> sig2(nmaps);
+  for (int c = 0; c < nmaps; c++) {
+    if (!(fwhm_arcmin[c] >= 0) || !std::isfinite(fwhm_arcmin[c]))
+      return sht_error(p, CRIME_GH_EINVAL, "fwhm[%d] = %g", c, fwhm_arcmin[c]);
+    const double sg = SHT_FWHM2SIGMA * fwhm_arcmin[c];
+    sig2[c] = sg * sg;
+  }
+  SHT_HIP(p, hipSetDevice(p->device));
+  rc = analyse(p, nmaps, d_maps, maps_fp64);
+  if (rc) return rc;
+  SHT_HIP(p, hipMemcpyAsync(p->d_sig2, sig2.data(), nmaps * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  SHT_HIP(p, launch_alm_beam(p->d_alm, p->lmax, nmaps, p->ldo, p->d_sig2, p->stream));
+  rc = synthesize(p, nmaps, d_maps, maps_fp64);
+  if (rc) return rc;
+  rc = finish(p, false);
+  if (rc) return rc;
+  const size_t nc = p->chunks.size();
+  hipEvent_t *ev = p->ev.data(), *sv = syn_events(p);
+  rc = finish_syn(p, true);
+  if (rc) return rc;
+  p->st.ms_cls = elapsed(ev[1 + 2 * nc], sv[0]);   // the beam
+  p->st.ms_total = elapsed(ev[0], sv[1 + 2 * nc]);
+  return 0;
+}
+
+extern "C" int crime_sht_add_noise_device(crime_sht_t *p, int nmaps, void *d_maps, int maps_fp64,
+                                          const double *sigma2, uint32_t seed)
+{
+  if (!p) return sht_error(nullptr, CRIME_GH_EINVAL, "plan is NULL");
+  if (nmaps < 1 || nmaps > p->max_maps)
+    return sht_error(p, CRIME_GH_EINVAL, "nmaps %d outside [1, %d]", nmaps, p->max_maps);
+  if (!d_maps || !sigma2) return sht_error(p, CRIME_GH_EINVAL, "maps or sigma2 are NULL");
+  for (int c = 0; c < nmaps; c++)
+    if (!(sigma2[c] >= 0) || !std::isfinite(sigma2[c]))
+      return sht_error(p, CRIME_GH_EINVAL, "sigma2[%d] = %g", c, sigma2[c]);
+  SHT_HIP(p, hipSetDevice(p->device));
+  SHT_HIP(p, hipMemcpyAsync(p->d_sig2, sigma2, nmaps * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  SHT_HIP(p, launch_noise(d_maps, maps_fp64, p->npix, nmaps, p->d_sig2, seed, p->stream));
+  SHT_HIP(p, hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+extern "C" int crime_sht_mask_device(crime_sht_t *p, int nmaps, void *d_maps, int maps_fp64, const uint8_t *mask)
+{
+  if (!p) return sht_error(nullptr, CRIME_GH_EINVAL, "plan is NULL");
+  if (nmaps < 1 || nmaps > p->max_maps)
+    return sht_error(p, CRIME_GH_EINVAL, "nmaps %d outside [1, %d]", nmaps, p->max_maps);
+  if (!d_maps || !mask) return sht_error(p, CRIME_GH_EINVAL, "maps or mask are NULL");
+  SHT_HIP(p, hipSetDevice(p->device));
+  if (!p->d_mask) SHT_HIP(p, hipMalloc((void **)&p->d_mask, (size_t)p->npix));
+  SHT_HIP(p, hipMemcpyAsync(p->d_mask, mask, (size_t)p->npix, hipMemcpyHostToDevice, p->stream));
+  SHT_HIP(p, launch_mask(d_maps, maps_fp64, p->npix, nmaps, p->d_mask, p->stream));
+  SHT_HIP(p, hipStreamSynchronize(p->stream));
+  return 0;
 }
 
 extern "C" int crime_sht_get_stats(const crime_sht_t *p, crime_sht_stats *st)

@@ -44,6 +44,8 @@ struct ShtLaunch {
   hipStream_t s;
 };
 hipError_t launch_rings(const ShtLaunch &a, const void *maps, int fp64);
+// synthesis: the same ring groups, from G into FP32 or FP64 maps
+hipError_t launch_syn_rings(const ShtLaunch &a, void *maps, int fp64);
 // B of the Bluestein rings [r0, r0 + nr) of one convolution length M
 hipError_t launch_blu_setup(int M, const ShtRing *rings, int r0, int nr, const int64_t *woff, const void *wtab,
                             const int64_t *boff, void *B, const void *tw, hipStream_t s);
@@ -60,6 +62,16 @@ hipError_t launch_alm_gemm(const double *lam, const double *G, const int4 *tiles
 hipError_t launch_cls(const double *alm, int lmax, int nmaps, int ldo, int cross, double *cls, hipStream_t s);
 hipError_t launch_cls_pairs(const double *alm, int lmax, int ldo, int npairs, const int *pairs, double *cls,
                             hipStream_t s);
+hipError_t launch_syn_gemm(const double *lam, const double *alm, double *G, const int4 *tiles, int ntiles,
+                           const int64_t *off, int m0, int lmax, int R, int RP, int ldg, int ncols, int ldo,
+                           hipStream_t s);
+hipError_t launch_alm_pack(const double *in, int64_t nalm, int nmaps, int ldo, double *alm, hipStream_t s);
+hipError_t launch_alm_beam(double *alm, int lmax, int nmaps, int ldo, const double *sig2, hipStream_t s);
+// second Philox key word of the JoinT-like noise ("JONT")
+constexpr uint32_t JT_NOISE_KEY = 0x4A4F4E54u;
+hipError_t launch_noise(void *maps, int fp64, int64_t npix, int nmaps, const double *sig2, uint32_t seed,
+                        hipStream_t s);
+hipError_t launch_mask(void *maps, int fp64, int64_t npix, int nmaps, const uint8_t *mask, hipStream_t s);
 hipError_t launch_alm_unpack(const double *alm, int64_t nalm, int nmaps, int ldo, double *out, hipStream_t s);
 
 }  // namespace crime

@@ -314,6 +314,145 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_ring_blu(const TM *__restrict
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Synthesis (alm2map) ring stage.  For a real map f_j = sum_{|m| <= lmax} F_m e^{i m phi_j} with
+// F_{-m} = conj F_m and phi_j = pi (2 j + s) / n, so f_j = sum_k D_k e^{+2 pi i jk/n} with the folded,
+// phase-shifted spectrum
+//   D_k = sum_{m = k mod n} F'_m + sum_{m > 0, -m = k mod n} conj F'_m,   F'_m = F_m e^{+i pi m s / n},
+// F_N = E + O and F_S = E - O from the synthesis GEMM (the same 16-ring blocks of G).  The north ring
+// and its mirror share one transform: z_k = D^N_k + i D^S_k gives f_N = Re Z, f_S = Im Z, both real
+// because D^N and D^S are Hermitian (Im a_l0 is dropped, as a real-map synthesis does).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ cx<double> syn_fold(const double *__restrict__ G, int r, int c, int lmax, int ldg, int n,
+                                               int s, bool eq, int k)
+{
+  const size_t rowoff = (size_t)(r & 15) * ldg + 2 * c;
+  double nr = 0, ni = 0, sr = 0, si = 0;
+  auto term = [&](int m, bool conj_) {
+    const double *ge = G + g_block(r >> 4, m, 0, lmax, ldg) + rowoff;
+    const double *go = ge + 16 * (size_t)ldg;
+    const double er = ge[0], ei = ge[1], orr = go[0], oi = go[1];
+    double fnr = er + orr, fni = ei + oi, fsr = er - orr, fsi = ei - oi;
+    if (m == 0) {
+      nr += fnr;
+      sr += fsr;
+      return;
+    }
+    double pr = 1, pi = 0;
+    if (s) {
+      const int tt = (int)((int64_t)m % (2 * n));
+      sincospi((double)tt / (double)n, &pi, &pr);
+    }
+    const double anr = fnr * pr - fni * pi, ani = fnr * pi + fni * pr;
+    const double asr = fsr * pr - fsi * pi, asi = fsr * pi + fsi * pr;
+    nr += anr;
+    sr += asr;
+    ni += conj_ ? -ani : ani;
+    si += conj_ ? -asi : asi;
+  };
+  for (int m = k; m <= lmax; m += n) term(m, false);
+  for (int m = k == 0 ? n : n - k; m <= lmax; m += n) term(m, true);
+  if (eq) sr = si = 0.0;
+  return {nr - si, ni + sr};
+}
+
+template <typename TM, int L>
+__global__ __launch_bounds__(RingFftCfg<L>::NT) void k_syn_fft(TM *__restrict__ maps, int64_t npix,
+                                                              const ShtRing *__restrict__ rings, int R, int lmax,
+                                                              int ldg, const double *__restrict__ G, int r0,
+                                                              int nmaps, int nlines, const cx<double> *__restrict__ tw)
+{
+  using C = RingFftCfg<L>;
+  __shared__ cx<double> lds[C::G * lpadded(L)];
+  const int g = threadIdx.x / C::TC, t = threadIdx.x % C::TC;
+  const int line = xcd_swizzle(blockIdx.x, gridDim.x) * C::G + g;
+  const bool live = line < nlines;            // padding lines still join every barrier
+  const int c = live ? line % nmaps : 0, r = r0 + (live ? line / nmaps : 0);
+  const ShtRing ri = rings[r];
+  const bool eq = (r == R - 1);
+  TM *fn = maps + (size_t)c * npix + ri.first_n;
+  TM *fs = maps + (size_t)c * npix + ri.first_s;
+  cx<double> *buf = lds + g * lpadded(L);
+  auto load = [&](int e) -> cx<double> {
+    if (!live) return {0.0, 0.0};
+    return syn_fold(G, r, c, lmax, ldg, L, ri.s, eq, e);
+  };
+  auto store = [&](int e, cx<double> v) {
+    if (!live) return;
+    fn[e] = (TM)v.x;
+    if (!eq) fs[e] = (TM)v.y;
+  };
+  auto slot = [&](int e) -> cx<double> & { return buf[lpad(e)]; };
+  Stockham<double, L, C::TC, C::MAXR>::run(t, load, store, slot, tw);
+}
+
+template <typename TM, int M>
+__global__ __launch_bounds__(BluCfg<M>::NT) void k_syn_blu(TM *__restrict__ maps, int64_t npix,
+                                                          const ShtRing *__restrict__ rings, int R, int lmax, int ldg,
+                                                          const double *__restrict__ G, int r0, int nmaps, int nlines,
+                                                          const int64_t *__restrict__ woff,
+                                                          const cx<double> *__restrict__ wtab,
+                                                          const int64_t *__restrict__ boff,
+                                                          const cx<double> *__restrict__ B,
+                                                          const cx<double> *__restrict__ tw)
+{
+  using C = BluCfg<M>;
+  __shared__ cx<double> lds[C::G * lpadded(M)];
+  const int g = threadIdx.x / C::TC, t = threadIdx.x % C::TC;
+  const int line = xcd_swizzle(blockIdx.x, gridDim.x) * C::G + g;
+  const bool live = line < nlines;
+  const int c = live ? line % nmaps : 0, r = r0 + (live ? line / nmaps : 0);
+  const ShtRing ri = rings[r];
+  const int n = ri.nphi;
+  const bool eq = (r == R - 1);
+  TM *fn = maps + (size_t)c * npix + ri.first_n;
+  TM *fs = maps + (size_t)c * npix + ri.first_s;
+  const cx<double> *w = wtab + woff[r];
+  const cx<double> *Bk = B + boff[r];
+  cx<double> *buf = lds + g * lpadded(M);
+  auto load1 = [&](int e) -> cx<double> {
+    if (!live || e >= n) return {0.0, 0.0};
+    return cmul(syn_fold(G, r, c, lmax, ldg, n, ri.s, eq, e), w[e]);
+  };
+  auto store1 = [&](int e, cx<double> v) { buf[lpad(e)] = v; };
+  auto slot = [&](int e) -> cx<double> & { return buf[lpad(e)]; };
+  Stockham<double, M, C::TC, C::MAXR>::run(t, load1, store1, slot, tw);
+  __syncthreads();
+  auto load2 = [&](int e) -> cx<double> { return cconj(cmul(buf[lpad(e)], Bk[e])); };
+  auto store2 = [&](int e, cx<double> v) {
+    if (!live || e >= n) return;
+    const cx<double> z = cmul(w[e], cconj(v));
+    fn[e] = (TM)z.x;
+    if (!eq) fs[e] = (TM)z.y;
+  };
+  Stockham<double, M, C::TC, C::MAXR>::template run<true>(t, load2, store2, slot, tw);
+}
+
+// rings shorter than 32 pixels: one workgroup per (map, ring pair), direct sums with exact phases
+template <typename TM>
+__global__ __launch_bounds__(64) void k_syn_dft(TM *__restrict__ maps, int64_t npix, const ShtRing *__restrict__ rings,
+                                                int R, int lmax, int ldg, const double *__restrict__ G, int r0)
+{
+  __shared__ cx<double> z[32];
+  const int c = blockIdx.x, r = r0 + blockIdx.y;
+  const ShtRing ri = rings[r];
+  const int n = ri.nphi;
+  const bool eq = (r == R - 1);
+  for (int k = threadIdx.x; k < n; k += 64) z[k] = syn_fold(G, r, c, lmax, ldg, n, ri.s, eq, k);
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += 64) {
+    double zr = 0, zi = 0;
+    for (int k = 0; k < n; k++) {
+      double sn, cs;
+      sincospi(2.0 * (double)((j * k) % n) / (double)n, &sn, &cs);
+      zr += z[k].x * cs - z[k].y * sn;
+      zi += z[k].x * sn + z[k].y * cs;
+    }
+    maps[(size_t)c * npix + ri.first_n + j] = (TM)zr;
+    if (!eq) maps[(size_t)c * npix + ri.first_s + j] = (TM)zi;
+  }
+}
+
 size_t ring_dft_lds_bytes(int n, int lmax)
 {
   int K = n < lmax + 1 ? n : lmax + 1;
@@ -584,6 +723,146 @@ __global__ __launch_bounds__(256) void k_alm_unpack(const double *__restrict__ a
   }
 }
 
+// G[ring][col] = sum_l lambda_lm(ring) alm[l][col] per (m, parity), l = m + parity + 2 row: the
+// transpose of the analysis product, into the same 16-ring blocks of G.  A workgroup computes
+// ALM_ROWS rings x GT columns, its 4 waves 2 x 2 blocks of (ALM_ROWS / 2) x 32; K runs over the
+// rows of this parity, 16 at a time.  Tiles with no rows (m = lmax, odd parity) write zeros.
+__global__ __launch_bounds__(256) void k_syn_gemm(const double *__restrict__ lam, const double *__restrict__ alm,
+                                                  double *__restrict__ G, const int4 *__restrict__ tiles, int ntiles,
+                                                  int ncoltiles, int tiles_per_xcd, const int64_t *__restrict__ off,
+                                                  int m0, int lmax, int R, int RP, int ldg, int ncols, int ldo)
+{
+  constexpr int RW = ALM_ROWS / 2, NI = RW / 16;
+  constexpr int AL = ALM_ROWS / 16;                 // rings loaded per thread
+  const int b = blockIdx.x;
+  const int logical = (b % 8) * tiles_per_xcd + b / 8;
+  if (logical >= ntiles * ncoltiles) return;
+  const int4 tl = tiles[logical / ncoltiles];
+  const int ct = logical % ncoltiles;
+  const int m = tl.x, par = tl.y, rt = tl.z, nrows = tl.w;
+  __shared__ double As[GK][ALM_ROWS + 2];   // [k][ring]: A stored as loaded, read down a column
+  __shared__ double Bs[GK][GT + 2];         // [k][column]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int ring0 = rt * ALM_ROWS, col0 = ct * GT;
+  const double *lrow = lam + (size_t)(off[m] - off[m0]) * RP;
+  const int64_t base_idx = (int64_t)m * lmax - (int64_t)m * (m - 1) / 2;   // he_indexlm(l, m) - l
+  dbl4 acc[NI][2];
+  for (int i = 0; i < NI; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0, 0, 0, 0};
+  const int ak = tid >> 4, ar = (tid & 15) * AL;     // A loads: row k, AL consecutive rings
+  const int bk = tid >> 4, bc = (tid & 15) * 4;      // B loads: row k, 4 columns
+  const bool aring_ok = ring0 + ar < RP;             // AL | RP: a chunk is wholly in range or not
+  const bool b0_ok = col0 + bc < ncols, b1_ok = col0 + bc + 2 < ncols;
+  const int bcol0 = b0_ok ? col0 + bc : 0, bcol1 = b1_ok ? col0 + bc + 2 : 0;
+  const int wr = (wv >> 1) * RW, wc = (wv & 1) * 32;
+  // operands out of range come from an in-range address and are zeroed by selects (no branches)
+  double2 ra[AL / 2], rb0, rb1;
+  bool a_ok = false, b_ok = false;
+  auto fetch = [&](int k0) {
+    a_ok = aring_ok && k0 + ak < nrows;
+    // row 0 of this m always exists (row `par` does not when m = lmax and the parity is odd)
+    const double *pa = lrow + (size_t)(a_ok ? par + 2 * (k0 + ak) : 0) * RP + (aring_ok ? ring0 + ar : 0);
+    for (int e = 0; e < AL / 2; e++) ra[e] = *(const double2 *)(pa + 2 * e);
+    b_ok = k0 + bk < nrows;
+    const int l = b_ok ? m + par + 2 * (k0 + bk) : m;
+    const double *pb = alm + (size_t)(m > 0 ? base_idx + l : l) * ldo;
+    rb0 = *(const double2 *)(pb + bcol0);
+    rb1 = *(const double2 *)(pb + bcol1);
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < nrows; k0 += GK) {
+    for (int e = 0; e < AL / 2; e++) {
+      As[ak][ar + 2 * e] = a_ok ? ra[e].x : 0.0;
+      As[ak][ar + 2 * e + 1] = a_ok ? ra[e].y : 0.0;
+    }
+    Bs[bk][bc] = b_ok && b0_ok ? rb0.x : 0.0;
+    Bs[bk][bc + 1] = b_ok && b0_ok ? rb0.y : 0.0;
+    Bs[bk][bc + 2] = b_ok && b1_ok ? rb1.x : 0.0;
+    Bs[bk][bc + 3] = b_ok && b1_ok ? rb1.y : 0.0;
+    __syncthreads();
+    fetch(k0 + GK < nrows ? k0 + GK : k0);
+#pragma unroll
+    for (int kk = 0; kk < GK; kk += 4) {
+      double af[NI], bf[2];
+      for (int i = 0; i < NI; i++) af[i] = As[kk + (lane >> 4)][wr + 16 * i + (lane & 15)];
+      for (int j = 0; j < 2; j++) bf[j] = Bs[kk + (lane >> 4)][wc + 16 * j + (lane & 15)];
+      for (int i = 0; i < NI; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int i = 0; i < NI; i++)
+    for (int j = 0; j < 2; j++)
+      for (int v = 0; v < 4; v++) {
+        const int ring = ring0 + wr + 16 * i + (lane >> 4) + 4 * v;
+        const int col = col0 + wc + 16 * j + (lane & 15);
+        if (ring < R && col < ncols)
+          G[g_block(ring >> 4, m, par, lmax, ldg) + (size_t)(ring & 15) * ldg + col] = acc[i][j][v];
+      }
+}
+
+// alms of the caller, [map][he_indexlm][re, im], into the plan's alm[idx][2 map + (re | im)]
+__global__ __launch_bounds__(256) void k_alm_pack(const double *__restrict__ in, int64_t nalm, int nmaps, int ldo,
+                                                  double *__restrict__ alm)
+{
+  const int t = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nalm; i += (int64_t)gridDim.x * 256) {
+    const double *src = in + ((size_t)t * nalm + i) * 2;
+    double *dst = alm + (size_t)i * ldo + 2 * t;
+    dst[0] = src[0];
+    dst[1] = src[1];
+  }
+}
+
+// he_alter_alm (src/healpix_extra.c:402-421): a_lm *= exp(-l (l + 1) sigma_c^2 / 2).  Block row m;
+// threads over (l, map) with the map fastest, so a wave touches consecutive alms of one row.
+__global__ __launch_bounds__(256) void k_alm_beam(double *__restrict__ alm, int lmax, int nmaps, int ldo,
+                                                  const double *__restrict__ sig2)
+{
+  const int m = blockIdx.y;
+  const int64_t base_idx = m > 0 ? (int64_t)m * lmax - (int64_t)m * (m - 1) / 2 : 0;
+  const int64_t n = (int64_t)(lmax - m + 1) * nmaps;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int l = m + (int)(i / nmaps), c = (int)(i % nmaps);
+    double *a = alm + (size_t)(base_idx + l) * ldo + 2 * c;
+    const double bl = exp(-0.5 * (double)l * (double)(l + 1) * sig2[c]);
+    a[0] *= bl;
+    a[1] *= bl;
+  }
+}
+
+// add_noise (src/main_jt.c:270-311), rng_delta_gauss (src/common.c:154-164): += sqrt(-sigma2 ln(1-u2))
+// cos(2 pi u1), (u1, u2) from Philox-4x32-10, counter (pixel, pixel >> 32, map, 0), key (seed, 'JONT')
+template <typename TM>
+__global__ __launch_bounds__(256) void k_noise(TM *__restrict__ maps, int64_t npix, int nmaps,
+                                               const double *__restrict__ sig2, uint32_t seed)
+{
+  const int64_t n = npix * nmaps;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int c = (int)(i / npix);
+    const int64_t p = i - (int64_t)c * npix;
+    u32x4 ctr;
+    ctr.v[0] = (uint32_t)p;
+    ctr.v[1] = (uint32_t)((uint64_t)p >> 32);
+    ctr.v[2] = (uint32_t)c;
+    ctr.v[3] = 0;
+    const u32x4 rr = philox4x32_10(ctr, seed, JT_NOISE_KEY);
+    const double u1 = u53(rr.v[0], rr.v[1]), u2 = u53(rr.v[2], rr.v[3]);
+    const double noise = sqrt(-sig2[c] * log(1.0 - u2)) * cos(2.0 * M_PI * u1);
+    maps[i] = (TM)((double)maps[i] + noise);
+  }
+}
+
+// apply_mask (src/main_jt.c:248-268)
+template <typename TM>
+__global__ __launch_bounds__(256) void k_mask(TM *__restrict__ maps, int64_t npix, int nmaps,
+                                              const uint8_t *__restrict__ mask)
+{
+  const int64_t n = npix * nmaps;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    if (!mask[i % npix]) maps[i] = (TM)0;
+}
+
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
@@ -613,6 +892,61 @@ static void launch_ring_blu_t(const ShtLaunch &a, const TM *maps)
   hipLaunchKernelGGL((k_ring_blu<TM, M>), dim3((nlines + C::G - 1) / C::G), dim3(C::NT), 0, a.s, maps, a.npix, a.rings,
                      a.R, a.RP, a.lmax, a.ldg, a.w, a.G, a.r0, a.nmaps, nlines, a.woff, (const cx<double> *)a.wtab,
                      a.boff, (const cx<double> *)a.B, (const cx<double> *)a.tw);
+}
+
+template <typename TM, int L>
+static void launch_syn_fft_t(const ShtLaunch &a, TM *maps)
+{
+  using C = RingFftCfg<L>;
+  const int nlines = a.nmaps * a.nr;
+  hipLaunchKernelGGL((k_syn_fft<TM, L>), dim3((nlines + C::G - 1) / C::G), dim3(C::NT), 0, a.s, maps, a.npix, a.rings,
+                     a.R, a.lmax, a.ldg, (const double *)a.G, a.r0, a.nmaps, nlines, (const cx<double> *)a.tw);
+}
+
+template <typename TM, int M>
+static void launch_syn_blu_t(const ShtLaunch &a, TM *maps)
+{
+  using C = BluCfg<M>;
+  const int nlines = a.nmaps * a.nr;
+  hipLaunchKernelGGL((k_syn_blu<TM, M>), dim3((nlines + C::G - 1) / C::G), dim3(C::NT), 0, a.s, maps, a.npix, a.rings,
+                     a.R, a.lmax, a.ldg, (const double *)a.G, a.r0, a.nmaps, nlines, a.woff,
+                     (const cx<double> *)a.wtab, a.boff, (const cx<double> *)a.B, (const cx<double> *)a.tw);
+}
+
+template <typename TM>
+static hipError_t launch_syn_t(const ShtLaunch &a, TM *maps)
+{
+  switch (a.kind) {
+  case SHT_RING_SPLIT:
+    hipLaunchKernelGGL(k_syn_dft<TM>, dim3(a.nmaps, a.nr), dim3(64), 0, a.s, maps, a.npix, a.rings, a.R, a.lmax, a.ldg,
+                       (const double *)a.G, a.r0);
+    break;
+  case SHT_RING_FFT:
+    switch (a.len) {
+#define CRIME_FFT(LL) \
+  case LL: launch_syn_fft_t<TM, LL>(a, maps); break;
+      CRIME_FOR_EACH_RING_FFT_L(CRIME_FFT)
+#undef CRIME_FFT
+    default: return hipErrorInvalidValue;
+    }
+    break;
+  case SHT_RING_BLUESTEIN:
+    switch (a.len) {
+#define CRIME_BLU(MM) \
+  case MM: launch_syn_blu_t<TM, MM>(a, maps); break;
+      CRIME_FOR_EACH_BLU_M(CRIME_BLU)
+#undef CRIME_BLU
+    default: return hipErrorInvalidValue;
+    }
+    break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_syn_rings(const ShtLaunch &a, void *maps, int fp64)
+{
+  return fp64 ? launch_syn_t<double>(a, (double *)maps) : launch_syn_t<float>(a, (float *)maps);
 }
 
 template <typename TM>
@@ -721,6 +1055,58 @@ hipError_t launch_cls_pairs(const double *alm, int lmax, int ldo, int npairs, co
                             hipStream_t s)
 {
   hipLaunchKernelGGL(k_cls_pairs, dim3(lmax + 1, npairs), dim3(64), 0, s, alm, lmax, ldo, pairs, cls);
+  return hipGetLastError();
+}
+
+hipError_t launch_syn_gemm(const double *lam, const double *alm, double *G, const int4 *tiles, int ntiles,
+                           const int64_t *off, int m0, int lmax, int R, int RP, int ldg, int ncols, int ldo,
+                           hipStream_t s)
+{
+  const int ncoltiles = (ncols + GT - 1) / GT;
+  const long total = (long)ntiles * ncoltiles;
+  const int per_xcd = (int)((total + 7) / 8);
+  hipLaunchKernelGGL(k_syn_gemm, dim3(8 * per_xcd), dim3(256), 0, s, lam, alm, G, tiles, ntiles, ncoltiles, per_xcd,
+                     off, m0, lmax, R, RP, ldg, ncols, ldo);
+  return hipGetLastError();
+}
+
+hipError_t launch_alm_pack(const double *in, int64_t nalm, int nmaps, int ldo, double *alm, hipStream_t s)
+{
+  int64_t nb = (nalm + 255) / 256;
+  dim3 grid((unsigned)(nb < 4096 ? nb : 4096), nmaps);
+  hipLaunchKernelGGL(k_alm_pack, grid, dim3(256), 0, s, in, nalm, nmaps, ldo, alm);
+  return hipGetLastError();
+}
+
+hipError_t launch_alm_beam(double *alm, int lmax, int nmaps, int ldo, const double *sig2, hipStream_t s)
+{
+  const int64_t per_m = (int64_t)(lmax + 1) * nmaps;   // the largest row (m = 0)
+  const int64_t bx = (per_m + 255) / 256;
+  dim3 grid((unsigned)(bx < 64 ? bx : 64), lmax + 1);
+  hipLaunchKernelGGL(k_alm_beam, grid, dim3(256), 0, s, alm, lmax, nmaps, ldo, sig2);
+  return hipGetLastError();
+}
+
+static unsigned grid_stride_blocks(int64_t n)
+{
+  int64_t b = (n + 255) / 256;
+  return (unsigned)(b < 65536 ? b : 65536);
+}
+
+hipError_t launch_noise(void *maps, int fp64, int64_t npix, int nmaps, const double *sig2, uint32_t seed,
+                        hipStream_t s)
+{
+  const dim3 grid(grid_stride_blocks(npix * nmaps));
+  if (fp64) hipLaunchKernelGGL(k_noise<double>, grid, dim3(256), 0, s, (double *)maps, npix, nmaps, sig2, seed);
+  else hipLaunchKernelGGL(k_noise<float>, grid, dim3(256), 0, s, (float *)maps, npix, nmaps, sig2, seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_mask(void *maps, int fp64, int64_t npix, int nmaps, const uint8_t *mask, hipStream_t s)
+{
+  const dim3 grid(grid_stride_blocks(npix * nmaps));
+  if (fp64) hipLaunchKernelGGL(k_mask<double>, grid, dim3(256), 0, s, (double *)maps, npix, nmaps, mask);
+  else hipLaunchKernelGGL(k_mask<float>, grid, dim3(256), 0, s, (float *)maps, npix, nmaps, mask);
   return hipGetLastError();
 }
 

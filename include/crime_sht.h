@@ -1,5 +1,6 @@
 /*
- * crime_sht.h -- HEALPix analysis (map2alm) and angular power spectra on one MI355X, in libcrime_gh.so.
+ * crime_sht.h -- HEALPix analysis (map2alm), synthesis (alm2map), angular power spectra and a JoinT-like
+ * beam / noise / mask stage on one MI355X, in libcrime_gh.so.
  *
  * Replaces he_map2alm (src/healpix_extra.c:88-130), which calls libsharp with
  * sharp_make_triangular_alm_info(lmax, lmax, 1) and sharp_make_weighted_healpix_geom_info(nside, 1,
@@ -64,7 +65,37 @@ int crime_sht_cls_pairs_device(crime_sht_t *p, int nmaps, const void *d_maps, in
 /* Same as crime_sht_anafast_device, for nmaps <= max_maps host maps maps[t][Npix] (FP64, RING). */
 int crime_sht_anafast(crime_sht_t *p, int nmaps, const double *maps, int cross, double *cls);
 
-/* Milliseconds of the last call's stages (ring DFTs, Legendre table, alm GEMMs, spectra). */
+/* he_alm2map (src/healpix_extra.c:44-86; libsharp SHARP_ALM2MAP with the same alm and geometry
+ * info as he_map2alm): for host alms[t][(lmax+1)(lmax+2)/2] (he_indexlm order, interleaved re, im) of
+ * nmaps <= max_maps real maps, maps[t][pix] = sum_{l, |m| <= l} a_lm Y_lm(pixel centre), with
+ * a_{l,-m} = (-1)^m conj a_lm; Im a_l0 is ignored, as for any real map.  maps: host FP64, RING. */
+int crime_sht_alm2map(crime_sht_t *p, int nmaps, const double *alms, double *maps);
+
+/* Same with device alms (same layout) into device maps d_maps[t * Npix + pix], FP64 (maps_fp64 = 1)
+ * or FP32 (maps_fp64 = 0). */
+int crime_sht_alm2map_device(crime_sht_t *p, int nmaps, const double *d_alms, void *d_maps, int maps_fp64);
+
+/* apply_beam of JoinT (src/main_jt.c:211-246) in place on device maps: map2alm, then
+ * he_alter_alm (src/healpix_extra.c:402-421) with the Gaussian window b_l = exp(-l(l+1) sigma_t^2 / 2),
+ * sigma_t = FWHM2SIGMA fwhm_arcmin[t] (src/healpix_extra.c:389-400), then alm2map.  fwhm_arcmin is a
+ * host array of nmaps values >= 0 (JoinT uses 1030610.55 / (nu_t [MHz] * dish diameter [m])). */
+int crime_sht_smooth_device(crime_sht_t *p, int nmaps, void *d_maps, int maps_fp64, const double *fwhm_arcmin);
+
+/* add_noise of JoinT (src/main_jt.c:270-311) on device maps: maps[t][pix] += sqrt(-sigma2[t] ln(1 - u2))
+ * cos(2 pi u1), the draw of rng_delta_gauss (src/common.c:154-164), whose variance is sigma2[t] / 2.
+ * (u1, u2) are 53-bit uniforms from Philox-4x32-10 with counter (pix, pix >> 32, t, 0) and key
+ * (seed, 0x4A4F4E54) -- the reference's per-thread GSL streams depend on its thread count, so no
+ * stream can match them; this one depends on nothing but (seed, t, pix).  sigma2: host, nmaps values. */
+int crime_sht_add_noise_device(crime_sht_t *p, int nmaps, void *d_maps, int maps_fp64, const double *sigma2,
+                               uint32_t seed);
+
+/* apply_mask of JoinT (src/main_jt.c:248-268): maps[t][pix] = 0 where mask[pix] == 0.  mask: host,
+ * Npix bytes (nonzero = observed). */
+int crime_sht_mask_device(crime_sht_t *p, int nmaps, void *d_maps, int maps_fp64, const uint8_t *mask);
+
+/* Milliseconds of the last call's stages: ring transforms, Legendre table, alm GEMMs, spectra (the beam
+ * for crime_sht_smooth_device), total.  A call with both an analysis and a synthesis reports the sum of
+ * the two for each stage. */
 typedef struct crime_sht_stats {
   double ms_rings, ms_legendre, ms_alm, ms_cls, ms_total;
 } crime_sht_stats;

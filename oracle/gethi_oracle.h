@@ -36,4 +36,7 @@ void orc_legendre(int lmax, int m, double x, double st, double *out);
 int64_t orc_alm_index(int l, int m, int lmax);
 int orc_map2alm(int64_t nside, int lmax, int nmaps, const double *maps, double *alms);
 void orc_alm2cl(int lmax, int nmaps, const double *alms, double *cls);
+int orc_alm2map(int64_t nside, int lmax, int nmaps, const double *alms, double *maps);
+void orc_beam_window(int lmax, double fwhm_arcmin, double *out);
+void orc_add_noise(int64_t npix, int nmaps, const double *sigma2, uint32_t seed, double *maps);
 #endif

@@ -62,6 +62,10 @@ def lib():
         L.orc_map2alm.argtypes = [I64, C.c_int, C.c_int, _dp, _dp]
         L.orc_map2alm.restype = C.c_int
         L.orc_alm2cl.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_alm2map.argtypes = [I64, C.c_int, C.c_int, _dp, _dp]
+        L.orc_alm2map.restype = C.c_int
+        L.orc_beam_window.argtypes = [C.c_int, C.c_double, _dp]
+        L.orc_add_noise.argtypes = [I64, C.c_int, _dp, C.c_uint32, _dp]
         _lib = L
     return _lib
 
@@ -222,3 +226,27 @@ def alm2cl(alms, lmax):
     out = np.empty((a.shape[0], a.shape[0], lmax + 1))
     lib().orc_alm2cl(lmax, a.shape[0], a.ctypes.data_as(_dp), _p(out))
     return out
+
+
+def alm2map(alms, nside, lmax):
+    """alms (nmaps, nalm) complex, m-major -> maps (nmaps, 12 nside^2) RING, by the defining sum."""
+    a = np.ascontiguousarray(np.atleast_2d(alms), dtype=np.complex128)
+    out = np.empty((a.shape[0], 12 * nside * nside))
+    rc = lib().orc_alm2map(nside, lmax, a.shape[0], a.ctypes.data_as(_dp), _p(out))
+    assert rc == 0
+    return out
+
+
+def beam_window(lmax, fwhm_arcmin):
+    """Gaussian beam b_l of he_generate_beam_window, l = 0..lmax."""
+    out = np.empty(lmax + 1)
+    lib().orc_beam_window(lmax, fwhm_arcmin, _p(out))
+    return out
+
+
+def add_noise(maps, sigma2, seed):
+    """JoinT's per-pixel noise added to a copy of maps (nmaps, npix): sqrt(-sigma2 ln(1-u2)) cos(2 pi u1)."""
+    m = np.array(np.atleast_2d(maps), dtype=np.float64, order="C")
+    s2 = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma2, np.float64), (m.shape[0],)))
+    lib().orc_add_noise(m.shape[1], m.shape[0], _p(s2), seed, _p(m))
+    return m

@@ -114,6 +114,84 @@ int orc_map2alm(int64_t nside, int lmax, int nmaps, const double *maps, double *
   return 0;
 }
 
+/* he_alm2map (src/healpix_extra.c:44-86, libsharp SHARP_ALM2MAP, unit weights): for a real map,
+ *   f(p) = sum_{l, |m| <= l} a_lm Y_lm(p) = sum_l a_l0 lambda_l0 + 2 Re sum_{m>0} F_m e^{i m phi_p},
+ *   F_m(ring) = sum_{l >= m} a_lm lambda_lm(z_ring),
+ * summed pixel by pixel with the exact phase m phi_j = pi t / nphi, t = m (2 j + s) mod 2 nphi.
+ * alms: nmaps x nalm complex (he_indexlm order); maps: nmaps x Npix (RING). */
+int orc_alm2map(int64_t nside, int lmax, int nmaps, const double *alms, double *maps)
+{
+  if (nside < 1 || lmax < 0 || nmaps < 1) return -1;
+  const int64_t npix = 12 * nside * nside, nring = 4 * nside - 1;
+  const int64_t nalm = (int64_t)(lmax + 1) * (lmax + 2) / 2;
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t ir = 1; ir <= nring; ir++) {
+    int64_t nphi, first;
+    double z, st;
+    int s;
+    orc_ring_info(nside, ir, &nphi, &first, &z, &st, &s);
+    double *lam = malloc(sizeof(double) * (size_t)(lmax + 1));
+    double *fm = malloc(sizeof(double) * 2 * (size_t)(lmax + 1));
+    for (int c = 0; c < nmaps; c++) {
+      const double *a = alms + 2 * (size_t)c * nalm;
+      for (int m = 0; m <= lmax; m++) {
+        orc_legendre(lmax, m, z, st, lam);
+        double re = 0, im = 0;
+        for (int l = m; l <= lmax; l++) {
+          const int64_t idx = orc_alm_index(l, m, lmax);
+          re += a[2 * idx] * lam[l - m];
+          im += a[2 * idx + 1] * lam[l - m];
+        }
+        fm[2 * m] = re;
+        fm[2 * m + 1] = im;
+      }
+      double *f = maps + (size_t)c * npix + first;
+      for (int64_t j = 0; j < nphi; j++) {
+        double v = fm[0];   /* Im a_l0 does not enter a real map */
+        for (int m = 1; m <= lmax; m++) {
+          int64_t t = ((int64_t)m * (2 * j + s)) % (2 * nphi);
+          double ang = M_PI * (double)t / (double)nphi;
+          v += 2.0 * (fm[2 * m] * cos(ang) - fm[2 * m + 1] * sin(ang));
+        }
+        f[j] = v;
+      }
+    }
+    free(lam);
+    free(fm);
+  }
+  return 0;
+}
+
+/* he_generate_beam_window (src/healpix_extra.c:389-400): b_l = exp(-l (l + 1) sigma^2 / 2),
+ * sigma = FWHM2SIGMA * fwhm_arcmin with FWHM2SIGMA = (pi / 10800) / sqrt(8 ln 2) */
+void orc_beam_window(int lmax, double fwhm_arcmin, double *out)
+{
+  const double sigma = (M_PI / 10800.0) / sqrt(8.0 * log(2.0)) * fwhm_arcmin;
+  for (int l = 0; l <= lmax; l++) out[l] = exp(-0.5 * (double)l * (double)(l + 1) * sigma * sigma);
+}
+
+/* 53-bit uniform from two Philox words, as the GPU library draws its modes */
+static double sht_u53(uint32_t hi, uint32_t lo)
+{
+  uint64_t m = ((uint64_t)hi << 21) ^ (uint64_t)(lo >> 11);
+  return (double)(m & ((1ull << 53) - 1)) / 9007199254740992.0;
+}
+
+/* add_noise (src/main_jt.c:270-311) with rng_delta_gauss (src/common.c:154-164):
+ *   maps[c][p] += sqrt(-sigma2[c] ln(1 - u2)) cos(2 pi u1),
+ * (u1, u2) from Philox-4x32-10 with counter (p, p >> 32, c, 0) and key (seed, 0x4A4F4E54), each a
+ * 53-bit uniform from two 32-bit words as the mode generator draws them. */
+void orc_add_noise(int64_t npix, int nmaps, const double *sigma2, uint32_t seed, double *maps)
+{
+  for (int c = 0; c < nmaps; c++)
+    for (int64_t p = 0; p < npix; p++) {
+      uint32_t ctr[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), (uint32_t)c, 0u}, r[4];
+      orc_philox4x32_10(ctr, seed, 0x4A4F4E54u, r);
+      double u1 = sht_u53(r[0], r[1]), u2 = sht_u53(r[2], r[3]);
+      maps[(size_t)c * npix + p] += sqrt(-sigma2[c] * log(1.0 - u2)) * cos(2.0 * M_PI * u1);
+    }
+}
+
 /* C_l^{ab} = (a_l0^a a_l0^b + 2 sum_{m>0} Re(a_lm^a conj a_lm^b)) / (2l + 1), for every pair (a, b):
  * cls[(a * nmaps + b) * (lmax + 1) + l] */
 void orc_alm2cl(int lmax, int nmaps, const double *alms, double *cls)

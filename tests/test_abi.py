@@ -98,6 +98,11 @@ def test_sht_null_and_bad_plans():
     h = C.c_void_p()
     assert lib.crime_sht_map2alm(None, 1, None, None) == -1
     assert lib.crime_sht_anafast_device(None, 1, None, 1, 1, None) == -1
+    assert lib.crime_sht_alm2map(None, 1, None, None) == -1
+    assert lib.crime_sht_alm2map_device(None, 1, None, None, 1) == -1
+    assert lib.crime_sht_smooth_device(None, 1, None, 1, None) == -1
+    assert lib.crime_sht_add_noise_device(None, 1, None, 1, None, 0) == -1
+    assert lib.crime_sht_mask_device(None, 1, None, 1, None) == -1
     lib.crime_sht_destroy(None)
     # bad geometry is rejected before any device work
     assert lib.crime_sht_create(0, 10, 1, 0, C.byref(h)) == -1

@@ -3,7 +3,10 @@
 Ring geometry: every pixel centre against Gorski et al. (2005) eqs. 4-9 as written in test_oracle.py;
 Legendre functions: scipy.special.sph_harm_y (l <= 80) and Unsold's theorem
 sum_m |Y_lm|^2 = (2l+1)/(4 pi) to l = 3000, polar rings included; map2alm: the defining sum
-(4 pi/Npix) sum_p f_p Y*_lm(p) with scipy's Y_lm at every pixel; alm2cl: the defining sum over m.
+(4 pi/Npix) sum_p f_p Y*_lm(p) with scipy's Y_lm at every pixel; alm2cl: the defining sum over m;
+alm2map: scipy's Y_lm at every pixel and the adjoint identity with the pinned map2alm; the beam window:
+the reference's FWHM2SIGMA literal; the JoinT noise: its distribution (variance sigma2 / 2, the
+reference's modulus-times-cosine draw) and its independence across pixels and channels.
 """
 import math
 
@@ -98,3 +101,75 @@ def test_alm2cl_defining_sum():
                 s = sum((alms[a, oracle.alm_index(l, m, lmax)] * np.conj(alms[b, oracle.alm_index(l, m, lmax)])
                          ).real * (1 if m == 0 else 2) for m in range(l + 1))
                 assert abs(cl[a, b, l] - s / (2 * l + 1)) < 1e-12 * abs(s)
+
+
+def _real_alms(rng, nm, lmax):
+    nalm = (lmax + 1) * (lmax + 2) // 2
+    a = rng.normal(size=(nm, nalm)) + 1j * rng.normal(size=(nm, nalm))
+    for l in range(lmax + 1):   # a real map has real a_l0
+        a[:, oracle.alm_index(l, 0, lmax)] = a[:, oracle.alm_index(l, 0, lmax)].real
+    return a
+
+
+@pytest.mark.parametrize("nside,lmax", [(1, 3), (2, 7), (4, 11), (4, 16), (3, 9)])
+def test_alm2map_defining_sum(nside, lmax):
+    """f(p) = sum_l a_l0 Y_l0 + 2 Re sum_{m>0} a_lm Y_lm(p) with scipy's Y_lm at every pixel centre."""
+    rng = np.random.default_rng(100 + nside + lmax)
+    alms = _real_alms(rng, 2, lmax)
+    got = oracle.alm2map(alms, nside, lmax)
+    th, ph = pixel_angles(nside)
+    ref = np.zeros_like(got)
+    for l in range(lmax + 1):
+        for m in range(l + 1):
+            y = sph_harm_y(l, m, th, ph)
+            term = (alms[:, oracle.alm_index(l, m, lmax)][:, None] * y[None, :]).real
+            ref += term if m == 0 else 2 * term
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("nside,lmax", [(8, 23), (16, 47), (16, 20)])
+def test_alm2map_adjoint_of_map2alm(nside, lmax):
+    """(4 pi/Npix) <f, alm2map(a)> = sum_lm c_m Re(a_lm conj map2alm(f)_lm), c_0 = 1, c_m = 2: the two
+    transforms are adjoint, so a slip in either (a phase sign, a dropped m, a parity) breaks it."""
+    rng = np.random.default_rng(nside * lmax)
+    npix = 12 * nside * nside
+    alms = _real_alms(rng, 1, lmax)
+    f = rng.normal(size=(1, npix))
+    lhs = (4 * np.pi / npix) * (f * oracle.alm2map(alms, nside, lmax)).sum()
+    af = oracle.map2alm(f, nside, lmax)
+    cm = np.concatenate([np.full(lmax - m + 1, 1.0 if m == 0 else 2.0) for m in range(lmax + 1)])
+    rhs = (cm * (alms[0] * np.conj(af[0])).real).sum()
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_beam_window_constant():
+    """he_generate_beam_window: the reference's FWHM2SIGMA literal (src/healpix_extra.c:389) equals
+    (pi/10800)/sqrt(8 ln 2); b_l = exp(-l(l+1) sigma^2/2) has b_0 = 1 and halves near the expected l."""
+    fwhm = 30.0
+    b = oracle.beam_window(3000, fwhm)
+    sigma = 0.00012352884853326381 * fwhm
+    ls = np.arange(3001)
+    assert np.abs(b - np.exp(-0.5 * ls * (ls + 1) * sigma * sigma)).max() < 1e-15
+    assert b[0] == 1.0
+    lh = np.searchsorted(-b, -0.5)
+    assert abs(lh * (lh + 1) * sigma * sigma - 2 * np.log(2)) < 2 * (2 * lh + 2) * sigma * sigma
+
+
+def test_noise_distribution():
+    """JoinT's draw sqrt(-sigma2 ln(1-u2)) cos(2 pi u1) is Gaussian with variance sigma2/2 (the modulus
+    has E[mod^2] = sigma2, E[cos^2] = 1/2); channels and pixels are independent draws."""
+    from scipy import stats
+    npix, s2 = 200000, np.array([4.0, 0.25])
+    n = oracle.add_noise(np.zeros((2, npix)), s2, 77)
+    for c in range(2):
+        z = n[c] / np.sqrt(s2[c] / 2)
+        assert abs(z.mean()) < 5 / np.sqrt(npix)
+        assert abs(z.var() - 1) < 5 * np.sqrt(2 / npix)
+        assert stats.kstest(z, "norm").pvalue > 1e-3
+    r = np.corrcoef(n[0] / 2, n[1] / 0.5)[0, 1]
+    assert abs(r) < 5 / np.sqrt(npix)
+    assert abs(np.corrcoef(n[0][:-1], n[0][1:])[0, 1]) < 5 / np.sqrt(npix)
+    # same seed, same noise; another seed, other noise; the input map is added to, not replaced
+    assert np.array_equal(oracle.add_noise(np.zeros((2, npix)), s2, 77), n)
+    assert not np.allclose(oracle.add_noise(np.zeros((2, npix)), s2, 78), n)
+    assert np.array_equal(oracle.add_noise(np.ones((2, npix)), s2, 77), n + 1)
