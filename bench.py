@@ -211,12 +211,65 @@ def bench_emulator(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_sht(args):
+    """Angular spectra and the JoinT beam on the 200 resident FP32 channels of the 1024^3 production mock
+    (Nside 512, lmax 1535).  A step is map2alm of every channel plus C_l of all 200 x 200 pairs, left in
+    HBM; the beam (map2alm, b_l, alm2map, in place) is timed as its own step after them."""
+    import numpy as np
+    import crime_amd
+    import crime_amd.configs as cf
+
+    with tempfile.TemporaryDirectory() as tmp:
+        par = crime_amd.Params(cf.production_1024(tmp))
+        with crime_amd.GetHI(par) as gh:
+            gh.run()
+            nside, lmax, nch = gh.n_side, 3 * gh.n_side - 1, gh.n_nu
+            nu0 = np.linspace(400.0, 798.0, nch)
+            fwhm, _ = crime_amd.jt_instrument(nu0, nu0 + 2.0, nside, dish_diameter=15.0)
+            with crime_amd.Sht(nside, lmax, max_maps=nch) as sh:
+                def spectra():
+                    sh.cls_of(gh, cross=True, to_host=False)
+                    return sh.stats()
+
+                def beam():
+                    sh.instrument_of(gh, fwhm_arcmin=fwhm)
+                    return sh.stats()
+
+                for _ in range(args.warmup):
+                    spectra()
+                    beam()
+                print("bench: warmup done", file=sys.stderr, flush=True)
+                cl_st = [spectra() for _ in range(args.steps)]
+                bm_st = [beam() for _ in range(args.steps)]
+    mean = lambda sts, k: sum(st[k] for st in sts) / len(sts)
+    ms_cl = mean(cl_st, "ms_total")
+    ms_gemm = mean(cl_st, "ms_alm")
+    nalm = (lmax + 1) * (lmax + 2) // 2
+    flops = 2.0 * nalm * (2 * nside) * (2 * nch)      # alm = Lambda G: rows x rings x (re, im) columns
+    achieved = flops / (ms_gemm * 1e-3) / 1e12
+    out = {
+        "metric": "maps/s map2alm + all-pair C_l at Nside 512", "value": nch / (ms_cl * 1e-3), "unit": "maps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_cl, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (the 200 FP32 channels of the 1024^3 production mock, resident in HBM)",
+        "config": {"workload": "sht_production", "n_side": nside, "lmax": lmax, "maps": nch,
+                   "pairs": nch * nch, "beam": "15 m dish, fwhm 1030610.55/(nu D) arcmin"},
+        "roofline": {"bound": "mfma", "kernel": "k_alm_gemm", "achieved": achieved, "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": achieved / 78.6, "traffic": None,
+                     "flop_per_launch": flops, "ms_per_launch": ms_gemm},
+        "stages_ms": {k: mean(cl_st, k) for k in ("ms_rings", "ms_legendre", "ms_alm", "ms_cls")},
+        "beam_ms": {k: mean(bm_st, k) for k in ("ms_rings", "ms_legendre", "ms_alm", "ms_cls", "ms_total")},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="production_1024", choices=WORKLOADS + ("ensemble_96", "emulator"))
+    ap.add_argument("--workload", default="production_1024",
+                    choices=WORKLOADS + ("ensemble_96", "emulator", "sht_production"))
     ap.add_argument("--emu-steps", type=int, default=20000, help="training steps of the emulator workload")
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel schedule (no autotune)")
     ap.add_argument("--streams", type=int, default=4, help="concurrent streams of the ensemble graph")
@@ -236,6 +289,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "emulator":
         return bench_emulator(args)
+    if args.workload == "sht_production":
+        return bench_sht(args)
 
     if args.calib_copy:
         import torch

@@ -520,6 +520,11 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int GT = 64, GK = 16;
 static_assert(GK == 16, "a GEMM k-step is one 16-ring block of G");
 static_assert(ALM_ROWS == 64 || ALM_ROWS == 128, "alm tiles of 64 or 128 rows");
+#ifndef CRIME_SYN_COLS
+#define CRIME_SYN_COLS 64
+#endif
+constexpr int SYN_COLS = CRIME_SYN_COLS;   // columns per synthesis GEMM tile (64 or 128)
+static_assert(SYN_COLS == 64 || SYN_COLS == 128, "synthesis tiles of 64 or 128 columns");
 
 // alm[l][col] = sum_r lambda_lm(r) G[r][col] per (m, parity): a workgroup computes ALM_ROWS x GT of
 // it, its 4 waves 2 x 2 blocks of (ALM_ROWS / 2) x 32 with 16x16x4 FP64 MFMAs.
@@ -725,15 +730,20 @@ __global__ __launch_bounds__(256) void k_alm_unpack(const double *__restrict__ a
 
 // G[ring][col] = sum_l lambda_lm(ring) alm[l][col] per (m, parity), l = m + parity + 2 row: the
 // transpose of the analysis product, into the same 16-ring blocks of G.  A workgroup computes
-// ALM_ROWS rings x GT columns, its 4 waves 2 x 2 blocks of (ALM_ROWS / 2) x 32; K runs over the
-// rows of this parity, 16 at a time.  Tiles with no rows (m = lmax, odd parity) write zeros.
+// ALM_ROWS rings x CT columns, its 4 waves 2 x 2 blocks of (ALM_ROWS / 2) x (CT / 2); K runs over the
+// rows of this parity, 16 at a time.  K averages a third of the analysis' (2 Nside rings), so the
+// wider tile keeps each workgroup's loop and its A tile busy for longer.  Tiles with no rows
+// (m = lmax, odd parity) write zeros.
+template <int CT>
 __global__ __launch_bounds__(256) void k_syn_gemm(const double *__restrict__ lam, const double *__restrict__ alm,
                                                   double *__restrict__ G, const int4 *__restrict__ tiles, int ntiles,
                                                   int ncoltiles, int tiles_per_xcd, const int64_t *__restrict__ off,
                                                   int m0, int lmax, int R, int RP, int ldg, int ncols, int ldo)
 {
   constexpr int RW = ALM_ROWS / 2, NI = RW / 16;
+  constexpr int CW = CT / 2, NJ = CW / 16;
   constexpr int AL = ALM_ROWS / 16;                 // rings loaded per thread
+  constexpr int BL = CT / 16;                       // columns loaded per thread
   const int b = blockIdx.x;
   const int logical = (b % 8) * tiles_per_xcd + b / 8;
   if (logical >= ntiles * ncoltiles) return;
@@ -741,22 +751,27 @@ __global__ __launch_bounds__(256) void k_syn_gemm(const double *__restrict__ lam
   const int ct = logical % ncoltiles;
   const int m = tl.x, par = tl.y, rt = tl.z, nrows = tl.w;
   __shared__ double As[GK][ALM_ROWS + 2];   // [k][ring]: A stored as loaded, read down a column
-  __shared__ double Bs[GK][GT + 2];         // [k][column]
+  __shared__ double Bs[GK][CT + 2];         // [k][column]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int ring0 = rt * ALM_ROWS, col0 = ct * GT;
+  const int ring0 = rt * ALM_ROWS, col0 = ct * CT;
   const double *lrow = lam + (size_t)(off[m] - off[m0]) * RP;
   const int64_t base_idx = (int64_t)m * lmax - (int64_t)m * (m - 1) / 2;   // he_indexlm(l, m) - l
-  dbl4 acc[NI][2];
+  dbl4 acc[NI][NJ];
   for (int i = 0; i < NI; i++)
-    for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0, 0, 0, 0};
+    for (int j = 0; j < NJ; j++) acc[i][j] = (dbl4){0, 0, 0, 0};
   const int ak = tid >> 4, ar = (tid & 15) * AL;     // A loads: row k, AL consecutive rings
-  const int bk = tid >> 4, bc = (tid & 15) * 4;      // B loads: row k, 4 columns
+  const int bk = tid >> 4, bc = (tid & 15) * BL;     // B loads: row k, BL consecutive columns
   const bool aring_ok = ring0 + ar < RP;             // AL | RP: a chunk is wholly in range or not
-  const bool b0_ok = col0 + bc < ncols, b1_ok = col0 + bc + 2 < ncols;
-  const int bcol0 = b0_ok ? col0 + bc : 0, bcol1 = b1_ok ? col0 + bc + 2 : 0;
-  const int wr = (wv >> 1) * RW, wc = (wv & 1) * 32;
+  // ncols is even and bc a multiple of 2: each double2 of B is wholly in range or not
+  bool bok[BL / 2];
+  int bcol[BL / 2];
+  for (int e = 0; e < BL / 2; e++) {
+    bok[e] = col0 + bc + 2 * e < ncols;
+    bcol[e] = bok[e] ? col0 + bc + 2 * e : 0;
+  }
+  const int wr = (wv >> 1) * RW, wc = (wv & 1) * CW;
   // operands out of range come from an in-range address and are zeroed by selects (no branches)
-  double2 ra[AL / 2], rb0, rb1;
+  double2 ra[AL / 2], rb[BL / 2];
   bool a_ok = false, b_ok = false;
   auto fetch = [&](int k0) {
     a_ok = aring_ok && k0 + ak < nrows;
@@ -766,8 +781,7 @@ __global__ __launch_bounds__(256) void k_syn_gemm(const double *__restrict__ lam
     b_ok = k0 + bk < nrows;
     const int l = b_ok ? m + par + 2 * (k0 + bk) : m;
     const double *pb = alm + (size_t)(m > 0 ? base_idx + l : l) * ldo;
-    rb0 = *(const double2 *)(pb + bcol0);
-    rb1 = *(const double2 *)(pb + bcol1);
+    for (int e = 0; e < BL / 2; e++) rb[e] = *(const double2 *)(pb + bcol[e]);
   };
   fetch(0);
   for (int k0 = 0; k0 < nrows; k0 += GK) {
@@ -775,24 +789,25 @@ __global__ __launch_bounds__(256) void k_syn_gemm(const double *__restrict__ lam
       As[ak][ar + 2 * e] = a_ok ? ra[e].x : 0.0;
       As[ak][ar + 2 * e + 1] = a_ok ? ra[e].y : 0.0;
     }
-    Bs[bk][bc] = b_ok && b0_ok ? rb0.x : 0.0;
-    Bs[bk][bc + 1] = b_ok && b0_ok ? rb0.y : 0.0;
-    Bs[bk][bc + 2] = b_ok && b1_ok ? rb1.x : 0.0;
-    Bs[bk][bc + 3] = b_ok && b1_ok ? rb1.y : 0.0;
+    for (int e = 0; e < BL / 2; e++) {
+      Bs[bk][bc + 2 * e] = b_ok && bok[e] ? rb[e].x : 0.0;
+      Bs[bk][bc + 2 * e + 1] = b_ok && bok[e] ? rb[e].y : 0.0;
+    }
     __syncthreads();
     fetch(k0 + GK < nrows ? k0 + GK : k0);
 #pragma unroll
     for (int kk = 0; kk < GK; kk += 4) {
-      double af[NI], bf[2];
+      double af[NI], bf[NJ];
       for (int i = 0; i < NI; i++) af[i] = As[kk + (lane >> 4)][wr + 16 * i + (lane & 15)];
-      for (int j = 0; j < 2; j++) bf[j] = Bs[kk + (lane >> 4)][wc + 16 * j + (lane & 15)];
+      for (int j = 0; j < NJ; j++) bf[j] = Bs[kk + (lane >> 4)][wc + 16 * j + (lane & 15)];
       for (int i = 0; i < NI; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NJ; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
   for (int i = 0; i < NI; i++)
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < NJ; j++)
       for (int v = 0; v < 4; v++) {
         const int ring = ring0 + wr + 16 * i + (lane >> 4) + 4 * v;
         const int col = col0 + wc + 16 * j + (lane & 15);
@@ -1062,11 +1077,17 @@ hipError_t launch_syn_gemm(const double *lam, const double *alm, double *G, cons
                            const int64_t *off, int m0, int lmax, int R, int RP, int ldg, int ncols, int ldo,
                            hipStream_t s)
 {
-  const int ncoltiles = (ncols + GT - 1) / GT;
+  // SYN_COLS-column tiles unless the columns fit one 64-column tile
+  const int ct = ncols > SYN_COLS / 2 ? SYN_COLS : 64;
+  const int ncoltiles = (ncols + ct - 1) / ct;
   const long total = (long)ntiles * ncoltiles;
   const int per_xcd = (int)((total + 7) / 8);
-  hipLaunchKernelGGL(k_syn_gemm, dim3(8 * per_xcd), dim3(256), 0, s, lam, alm, G, tiles, ntiles, ncoltiles, per_xcd,
-                     off, m0, lmax, R, RP, ldg, ncols, ldo);
+  if (ct == 64)
+    hipLaunchKernelGGL(k_syn_gemm<64>, dim3(8 * per_xcd), dim3(256), 0, s, lam, alm, G, tiles, ntiles, ncoltiles,
+                       per_xcd, off, m0, lmax, R, RP, ldg, ncols, ldo);
+  else
+    hipLaunchKernelGGL(k_syn_gemm<SYN_COLS>, dim3(8 * per_xcd), dim3(256), 0, s, lam, alm, G, tiles, ntiles,
+                       ncoltiles, per_xcd, off, m0, lmax, R, RP, ldg, ncols, ldo);
   return hipGetLastError();
 }
 
