@@ -209,6 +209,30 @@ class GetHI:
         self._check(self._lib.crime_gh_device_maps(self._h, C.byref(p), C.byref(nb)))
         return p.value, nb.value
 
+    def mk_psources_maps(self):
+        """Point sources (do_psources): Poisson counts per cell, positions, luminosities and the
+        point-source maps, on the resident fields.  Returns timings and the number of sources."""
+        st = _ffi.CrimeGhPsStats()
+        self._check(self._lib.crime_gh_mk_psources_maps(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def ps_maps(self):
+        """Point-source maps (n_nu, Npix), mK, FP64 copy."""
+        out = np.empty((self.n_nu, self.npix), np.float64)
+        self._check(self._lib.crime_gh_copy_ps_maps(self._h, _dptr(out)))
+        return out
+
+    def ps_pixel_sums(self):
+        """The two per-pixel terms (A, B) of the last point-source pass."""
+        a, b = np.empty(self.npix), np.empty(self.npix)
+        self._check(self._lib.crime_gh_ps_pixel_sums(self._h, _dptr(a), _dptr(b)))
+        return a, b
+
+    def device_ps_maps(self):
+        p, nb = C.c_void_p(), C.c_size_t()
+        self._check(self._lib.crime_gh_device_ps_maps(self._h, C.byref(p), C.byref(nb)))
+        return p.value, nb.value
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.crime_gh_destroy(self._h)

@@ -83,6 +83,15 @@ class CrimeShtStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class CrimeGhPsStats(C.Structure):
+    """Mirror of ``crime_gh_ps_stats``."""
+
+    _fields_ = [("ms_sources", C.c_double), ("ms_maps", C.c_double), ("n_sources", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class CrimeGhStats(C.Structure):
     """Mirror of ``crime_gh_stats``."""
 
@@ -158,6 +167,14 @@ def gpu_lib():
         lib.crime_gh_power_spectrum.argtypes = [H, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int64)]
         lib.crime_gh_device_maps.restype = C.c_int
         lib.crime_gh_device_maps.argtypes = [H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
+        lib.crime_gh_mk_psources_maps.restype = C.c_int
+        lib.crime_gh_mk_psources_maps.argtypes = [H, C.POINTER(CrimeGhPsStats)]
+        lib.crime_gh_copy_ps_maps.restype = C.c_int
+        lib.crime_gh_copy_ps_maps.argtypes = [H, _dp]
+        lib.crime_gh_device_ps_maps.restype = C.c_int
+        lib.crime_gh_device_ps_maps.argtypes = [H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
+        lib.crime_gh_ps_pixel_sums.restype = C.c_int
+        lib.crime_gh_ps_pixel_sums.argtypes = [H, _dp, _dp]
         lib.crime_gh_run_batch.restype = C.c_int
         lib.crime_gh_run_batch.argtypes = [C.POINTER(H), C.c_int, C.POINTER(CrimeGhStats)]
         lib.crime_gh_batch_create.restype = C.c_int

@@ -44,6 +44,11 @@ struct crime_gh {
   bool fields_done = false;
   Schedule sched{};            // kernel variants (crime_gh_autotune / crime_gh_set_schedule)
   hipEvent_t ev[8] = {};
+  // point sources (allocated by the first crime_gh_mk_psources_maps)
+  double *d_ps = nullptr;      // nz[PS_NZ] | maxpdf[PS_NZ] | c1[n_nu] | c2[n_nu] | A[npix] | B[npix]
+  unsigned long long *d_ps_count = nullptr;
+  void *ps_maps = nullptr;     // [n_nu * npix], the grid's precision, mK
+  std::vector<double> ps_nu;   // channel centres of mk_psources_maps (pixelize.c:62-69)
   std::string err;
 };
 
@@ -176,7 +181,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 static int free_all(crime_gh *h)
 {
   void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_guess, h->d_amp, h->d_order, h->d_sub,
-                  h->d_modes, h->d_tw, h->d_partials, h->d_sig};
+                  h->d_modes, h->d_tw, h->d_partials, h->d_sig, h->d_ps, h->d_ps_count, h->ps_maps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_sig) (void)hipHostFree(h->h_sig);
@@ -273,6 +278,10 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     double r = r_of_z_host(d, CRIME_GH_NU_21 / nu - 1);
     prefac[i] = m2t / (r * r * dnu);
   }
+  h->ps_nu.resize(d->n_nu);
+  for (int i = 0; i < d->n_nu; i++)
+    h->ps_nu[i] = d->nu_mode == CRIME_GH_NU_TABLE ? (d->nu0[i] + d->nuf[i]) * 0.5
+                                                  : d->nu_min + (d->nu_max - d->nu_min) * (i + 0.5) / d->n_nu;
   HIP_OR_FAIL(h, hipMemcpy(h->d_edges, edges.data(), edges.size() * sizeof(double), hipMemcpyHostToDevice));
   HIP_OR_FAIL(h, hipMemcpy(h->d_prefac, prefac.data(), prefac.size() * sizeof(double), hipMemcpyHostToDevice));
   // channel guess table: guess[j] = channel holding g_lo + j/g_inv (table mode)
@@ -758,6 +767,128 @@ extern "C" int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes)
   if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
   if (dptr) *dptr = h->maps;
   if (bytes) *bytes = h->nmaps * (h->prec ? sizeof(double) : sizeof(float));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// point sources
+// ---------------------------------------------------------------------------------------------
+// setup_psources (src/psources.c:112-145): n(z_i) = sum over 256 log10 L midpoints of dlogL
+// l_z_function, and max_Lpdf = 1.1 max l_z_function / n(z_i), at z_i = i z_max / 256
+static double ps_lfunc_host(double L, double z)
+{
+  const double rho_l = 2.5 * M_LN10 * PS_RHO * pow(L / PS_LX, PS_ALPHA + 1) * exp(-L / PS_LX);
+  const double fz = z < 1.5 ? pow(1 + z, 3.1) : 17.124;
+  return fz * rho_l;
+}
+
+static void ps_tables(double z_max, double *nz, double *maxpdf)
+{
+  const double dlogL = (PS_LLOGMAX - PS_LLOGMIN) / PS_NL, dz = z_max / PS_NZ;
+  for (int ii = 0; ii < PS_NZ; ii++) {
+    const double z = ii * dz;
+    double acc = 0, mx = -1;
+    for (int jj = 0; jj < PS_NL; jj++) {
+      const double lf = ps_lfunc_host(pow(10., PS_LLOGMIN + (jj + 0.5) * dlogL), z);
+      if (lf >= mx) mx = lf;
+      acc += dlogL * lf;
+    }
+    nz[ii] = acc;
+    maxpdf[ii] = acc > 0 ? 1.1 * mx / acc : 0;
+  }
+}
+
+extern "C" int crime_gh_mk_psources_maps(crime_gh_t *h, crime_gh_ps_stats *st)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (!h->fields_done) return set_error(h, CRIME_GH_ESTATE, "mk_psources_maps before create_d_and_vr_fields");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  const size_t npix = (size_t)h->npix, nps = 2 * PS_NZ + 2 * (size_t)h->n_nu + 2 * npix;
+  const size_t esz = h->prec ? sizeof(double) : sizeof(float);
+  if (!h->d_ps) {
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_ps, nps * sizeof(double)));
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_ps_count, sizeof(unsigned long long)));
+    HIP_OR_FAIL(h, hipMalloc(&h->ps_maps, (size_t)h->n_nu * npix * esz));
+    // tables and channel factors: T(pix, nu) = K (A nu^-2.1 + B nu^-2.75), K = FLUX2TEMP LUM2FLUX
+    // 4 pi SED_NORM h^2 / dOmega, nu the channel centre (pixelize.c:62-69)
+    std::vector<double> host(2 * PS_NZ + 2 * (size_t)h->n_nu);
+    const double z_max = CRIME_GH_NU_21 / h->d.nu_min - 1;   // cosmo.c:357
+    ps_tables(z_max, host.data(), host.data() + PS_NZ);
+    const double domega = 4 * M_PI / (double)npix;
+    const double K = PS_FLUX2TEMP * PS_LUM2FLUX * 4 * M_PI * PS_SED_NORM * h->d.hhub * h->d.hhub / domega;
+    for (int i = 0; i < h->n_nu; i++) {
+      const double nu = h->ps_nu[i];
+      host[2 * PS_NZ + i] = K * pow(nu, -2.1);
+      host[2 * PS_NZ + h->n_nu + i] = K * pow(nu, -2.75);
+    }
+    HIP_OR_FAIL(h, hipMemcpy(h->d_ps, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  PsParams ps{};
+  ps.nz = h->d_ps;
+  ps.maxpdf = h->d_ps + PS_NZ;
+  ps.z_max = CRIME_GH_NU_21 / h->d.nu_min - 1;
+  ps.dz = ps.z_max / PS_NZ;
+  ps.inv_dz = 1.0 / ps.dz;
+  ps.cell_vol = h->map.dx * h->map.dx * h->map.dx;
+  ps.seed = h->d.seed;
+  ps.A = h->d_ps + 2 * PS_NZ + 2 * (size_t)h->n_nu;
+  ps.B = ps.A + npix;
+  ps.count = h->d_ps_count;
+  const double *c1 = h->d_ps + 2 * PS_NZ, *c2 = c1 + h->n_nu;
+  HIP_OR_FAIL(h, hipEventRecord(h->ev[0], h->stream));
+  HIP_OR_FAIL(h, hipMemsetAsync(ps.A, 0, 2 * npix * sizeof(double), h->stream));
+  HIP_OR_FAIL(h, hipMemsetAsync(h->d_ps_count, 0, sizeof(unsigned long long), h->stream));
+  hipError_t e = h->prec ? launch_psources<double>((const cx<double> *)h->F, h->map, ps, h->stream)
+                         : launch_psources<float>((const cx<float> *)h->F, h->map, ps, h->stream);
+  HIP_OR_FAIL(h, e);
+  HIP_OR_FAIL(h, hipEventRecord(h->ev[1], h->stream));
+  e = h->prec ? launch_ps_expand<double>(ps.A, ps.B, c1, c2, h->n_nu, h->npix, (double *)h->ps_maps, h->stream)
+              : launch_ps_expand<float>(ps.A, ps.B, c1, c2, h->n_nu, h->npix, (float *)h->ps_maps, h->stream);
+  HIP_OR_FAIL(h, e);
+  HIP_OR_FAIL(h, hipEventRecord(h->ev[2], h->stream));
+  HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+  if (st) {
+    unsigned long long nsrc = 0;
+    HIP_OR_FAIL(h, hipMemcpy(&nsrc, h->d_ps_count, sizeof(nsrc), hipMemcpyDeviceToHost));
+    st->n_sources = (int64_t)nsrc;
+    st->ms_sources = ev_ms(h->ev[0], h->ev[1]);
+    st->ms_maps = ev_ms(h->ev[1], h->ev[2]);
+  }
+  return 0;
+}
+
+extern "C" int crime_gh_copy_ps_maps(crime_gh_t *h, double *out)
+{
+  if (!h || !out) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (!h->ps_maps) return set_error(h, CRIME_GH_ESTATE, "no point-source maps yet");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  const size_t n = (size_t)h->n_nu * h->npix;
+  if (h->prec) {
+    HIP_OR_FAIL(h, hipMemcpy(out, h->ps_maps, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<float> tmp(n);
+  HIP_OR_FAIL(h, hipMemcpy(tmp.data(), h->ps_maps, n * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; i++) out[i] = tmp[i];
+  return 0;
+}
+
+extern "C" int crime_gh_ps_pixel_sums(crime_gh_t *h, double *A, double *B)
+{
+  if (!h || !A || !B) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (!h->d_ps) return set_error(h, CRIME_GH_ESTATE, "no point sources yet");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  const double *dA = h->d_ps + 2 * PS_NZ + 2 * (size_t)h->n_nu;
+  HIP_OR_FAIL(h, hipMemcpy(A, dA, (size_t)h->npix * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OR_FAIL(h, hipMemcpy(B, dA + h->npix, (size_t)h->npix * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int crime_gh_device_ps_maps(crime_gh_t *h, void **dptr, size_t *bytes)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (dptr) *dptr = h->ps_maps;
+  if (bytes) *bytes = h->ps_maps ? (size_t)h->n_nu * h->npix * (h->prec ? sizeof(double) : sizeof(float)) : 0;
   return 0;
 }
 

@@ -143,6 +143,33 @@ struct MapParams {
 constexpr int TILE_X = 16, TILE_Y = 4, TILE_Z = 4;
 constexpr int AGG_SLOTS = 2048;
 
+// Point sources (src/grid_tools.c:24-101, src/pixelize.c:58-148, src/psources.c): constants of the
+// reference's toy luminosity function and SED (psources.c:23-63, 147-152) and the per-run tables.
+constexpr int PS_NZ = 256;                      // NZ_PSOURCES (src/common_gh.h:35)
+constexpr double PS_LLOGMIN = -5.0, PS_LLOGMAX = 6.0;
+constexpr int PS_NL = 256;                      // NL_PSOURCES
+constexpr double PS_LX = 2.11, PS_ALPHA = -0.633, PS_RHO = 3.2e-4;
+constexpr double PS_SED_NORM = 0.114461, PS_FLUX2TEMP = 3.2548291e-2, PS_LUM2FLUX = 8.35774e7;
+constexpr int PS_MAX_TRIES = 4096;              // cap of the luminosity rejection loop
+constexpr uint32_t PS_KEY1 = 0x50535243u;       // second Philox key word ("PSRC")
+constexpr int PS_SLOTS = 1024;                  // pixels of a tile's LDS table (24 KB)
+
+struct PsParams {
+  const double *nz;          // [PS_NZ] n(z_i) (nz_psources_arr)
+  const double *maxpdf;      // [PS_NZ] (max_Lpdf_arr)
+  double dz, inv_dz, z_max;  // glob_dz, glob_inv_dz, z_max (psources.c:117-118, cosmo.c:357)
+  double cell_vol;
+  uint32_t seed;
+  double *A, *B;             // [npix] sums of the two SED terms per pixel
+  unsigned long long *count; // sources drawn
+};
+
+template <typename T>
+hipError_t launch_psources(const cx<T> *F, const MapParams &m, const PsParams &ps, hipStream_t s);
+template <typename T>
+hipError_t launch_ps_expand(const double *A, const double *B, const double *c1, const double *c2, int n_nu,
+                            int64_t npix, T *maps, hipStream_t s);
+
 template <typename T>
 hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
                          double *partials, double *sig, hipStream_t s, hipEvent_t *ev, const Schedule &sc);

@@ -188,6 +188,237 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Point sources.  get_point_sources (grid_tools.c:24-101) draws a Poisson number of sources per cell
+// from the Gaussian delta; mk_psources_maps (pixelize.c:58-148) places each uniformly in its cell,
+// draws its luminosity from the toy luminosity function by rejection (psources.c:147-170) and adds
+//   T(nu) = FLUX2TEMP LUM2FLUX 4 pi L SED((1+z) nu) h^2 / (r^2 (1+z) dOmega nu^2)      (:173-184)
+// to every channel of its pixel.  SED = SED_NORM (x^-0.1 + 10 x^-0.75), x = (1+z) nu / 1000, factors
+// into a source part and a channel part, so each source adds two numbers to its pixel,
+//   A += L/(r^2 (1+z)) ((1+z)/1000)^-0.1,   B += 10 L/(r^2 (1+z)) ((1+z)/1000)^-0.75,
+// and T(pix, nu) = K (A nu^-2.1 + B nu^-2.75) afterwards: n_nu times fewer atomics than the
+// reference's per-channel adds, the same sum up to rounding.
+// Random numbers: Philox-4x32-10 with key (seed, PS_KEY1) and counter (cell, cell >> 32, w2, w3):
+// Poisson draws w2 = 0, w3 = j; source s's position w2 = 4 s + 1 (w3 = 0: x, y; 1: z); its luminosity
+// w2 = 4 s + 2, w3 = attempt.  The reference's per-thread GSL streams depend on its thread count.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void ps_uniforms(uint32_t seed, uint64_t cell, uint32_t w2, uint32_t w3, double &u1,
+                                            double &u2)
+{
+  u32x4 c;
+  c.v[0] = (uint32_t)cell;
+  c.v[1] = (uint32_t)(cell >> 32);
+  c.v[2] = w2;
+  c.v[3] = w3;
+  const u32x4 r = philox4x32_10(c, seed, PS_KEY1);
+  u1 = u53(r.v[0], r.v[1]);
+  u2 = u53(r.v[2], r.v[3]);
+}
+
+// Poisson(lambda): inversion by sequential search below 10, PTRS (Hormann 1993, Insurance: Math. and
+// Econ. 12, 39) above.  Both loops are bounded; the bounds are never reached in practice.
+__device__ uint32_t ps_poisson(double lam, uint32_t seed, uint64_t cell)
+{
+  if (!(lam > 0)) return 0;
+  if (lam < 10.0) {
+    double u, unused;
+    ps_uniforms(seed, cell, 0, 0, u, unused);
+    double p = exp(-lam), F = p;
+    uint32_t k = 0;
+    while (u > F && k < 1000) {
+      k++;
+      p *= lam / (double)k;
+      F += p;
+    }
+    return k;
+  }
+  const double slam = sqrt(lam), loglam = log(lam);
+  const double b = 0.931 + 2.53 * slam, a = -0.059 + 0.02483 * b;
+  const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4), vr = 0.9277 - 3.6224 / (b - 2.0);
+  for (uint32_t j = 0; j < 1024; j++) {
+    double U, V;
+    ps_uniforms(seed, cell, 0, j, U, V);
+    U -= 0.5;
+    const double us = 0.5 - fabs(U);
+    const double k = floor((2.0 * a / us + b) * U + lam + 0.43);
+    if (us >= 0.07 && V <= vr) return (uint32_t)k;
+    if (k < 0 || (us < 0.013 && V > us)) continue;
+    if (log(V) + log(inv_alpha) - log(a / (us * us) + b) <= -lam + k * loglam - lgamma(k + 1.0)) return (uint32_t)k;
+  }
+  return (uint32_t)lam;
+}
+
+// n_of_z_psources (psources.c:78-95)
+__device__ __forceinline__ double ps_nofz(const PsParams &p, double z)
+{
+  const int iz = (int)(z * p.inv_dz);
+  if (iz >= PS_NZ || iz < 0) return -1;
+  if (iz == PS_NZ - 1) return p.nz[PS_NZ - 1];
+  const double zi = iz * p.dz;
+  return p.nz[iz] + (p.nz[iz + 1] - p.nz[iz]) * (z - zi) * p.inv_dz;
+}
+
+// draw_luminosity (psources.c:147-170): u, then log10 L uniform in [LLOGMIN, LLOGMAX], accepted when
+// u max_pdf < l_z_function(L, z) / n(z); z >= z_max gives L = 0.  10^x = exp(x ln 10) and
+// (L/LX)^(alpha+1) = exp((alpha+1) ln(L/LX)) are the same functions written with one exponential each.
+// ps_lum_setup: the per-source constants (false when z >= z_max); ps_lum_try: attempt `att`.
+__device__ __forceinline__ bool ps_lum_setup(const PsParams &p, double z, double &maxpdf, double &c)
+{
+  if (z >= p.z_max) return false;
+  int iz = (int)(z * p.inv_dz);
+  iz = iz < 0 ? 0 : (iz < PS_NZ ? iz : PS_NZ - 1);   // z just below z_max can round to 256
+  maxpdf = p.maxpdf[iz];
+  const double fz = z < 1.5 ? pow(1.0 + z, 3.1) : 17.124;
+  c = fz * 2.5 * M_LN10 * PS_RHO / ps_nofz(p, z);
+  return true;
+}
+
+__device__ __forceinline__ bool ps_lum_try(uint32_t seed, uint64_t cell, uint32_t src, uint32_t att, double maxpdf,
+                                           double c, double &L)
+{
+  double u, w;
+  ps_uniforms(seed, cell, 4 * src + 2, att, u, w);
+  const double logL = PS_LLOGMIN + (PS_LLOGMAX - PS_LLOGMIN) * w;
+  L = exp(logL * M_LN10);
+  const double x = L / PS_LX;
+  return u * maxpdf < c * exp((PS_ALPHA + 1.0) * log(x) - x);
+}
+
+// Add (a, b) to pixel `key` of the workgroup's LDS table (open addressing, linear probing); a source
+// that finds no slot within the probe budget goes straight to A, B in HBM.
+template <int SLOTS>
+__device__ __forceinline__ void ps_agg_add(unsigned long long key, double a, double b, unsigned long long *keys,
+                                           double *va, double *vb, double *__restrict__ A, double *__restrict__ B)
+{
+  const unsigned long long EMPTY = ~0ull;
+  constexpr int SHIFT = 64 - __builtin_ctz(SLOTS);
+  unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> SHIFT) & (SLOTS - 1);
+  for (int p = 0; p < 64; p++) {
+    unsigned long long k = keys[h];
+    if (k == EMPTY) {
+      unsigned long long old = atomicCAS(&keys[h], EMPTY, key);
+      if (old == EMPTY || old == key) {
+        atomicAdd(&va[h], a);
+        atomicAdd(&vb[h], b);
+        return;
+      }
+    }
+    else if (k == key) {
+      atomicAdd(&va[h], a);
+      atomicAdd(&vb[h], b);
+      return;
+    }
+    h = (h + 1) & (SLOTS - 1);
+  }
+  unsafeAtomicAdd(A + key, a);
+  unsafeAtomicAdd(B + key, b);
+}
+
+// One workgroup per map-stage tile (TILE_X x TILE_Y x TILE_Z cells, one per thread, tiles in order of
+// distance from the observer).  A thread merges consecutive sources of its cell that share a pixel,
+// the workgroup sums per pixel in LDS, and each occupied pixel is added to A, B once.
+template <typename T>
+__global__ __launch_bounds__(256) void k_psources(const cx<T> *__restrict__ F, MapParams m, PsParams ps)
+{
+  constexpr int SLOTS = PS_SLOTS;
+  __shared__ unsigned long long keys[SLOTS];
+  __shared__ double va[SLOTS], vb[SLOTS];
+  __shared__ unsigned long long red[256];
+  const int n = m.n;
+  const int t = threadIdx.x;
+  const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (bid >= (size_t)m.ntx * m.nty * m.ntz) return;   // whole workgroup
+  const size_t tb = m.tile_order[bid];
+  const int tx = (int)(tb % m.ntx), ty = (int)((tb / m.ntx) % m.nty), tz = (int)(tb / ((size_t)m.ntx * m.nty));
+  const int ix = tx * TILE_X + (t % TILE_X), iy = ty * TILE_Y + ((t / TILE_X) % TILE_Y),
+            iz = tz * TILE_Z + t / (TILE_X * TILE_Y);
+  for (int i = t; i < SLOTS; i += 256) {
+    keys[i] = ~0ull;
+    va[i] = 0.0;
+    vb[i] = 0.0;
+  }
+  __syncthreads();
+  unsigned long long mine = 0;
+  if (ix < n && iy < n && iz < n) {
+    const size_t cell = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
+    // get_point_sources: cell centre, with the reference's x offset by pos_obs[1] (grid_tools.c:66)
+    const double xc = (ix + 0.5) * m.dx - m.obs[1];
+    const double yc = (iy + 0.5) * m.dx - m.obs[1];
+    const double zc = (iz + 0.5) * m.dx - m.obs[2];
+    const double rc = sqrt(xc * xc + yc * yc + zc * zc);
+    const double zred = z_of_r(m.tab, rc);
+    const double ndens = ps_nofz(ps, zred);
+    uint32_t np = 0;
+    if (ndens > 0) {
+      const double gfb = dgrowth_of_r(m.tab, rc);   // bias_psources = 1 (psources.c:68-71)
+      const double lam = ndens * ps.cell_vol * exp(gfb * ((double)F[cell].x - 0.5 * gfb * m.sig[0]));
+      np = ps_poisson(lam, ps.seed, cell);
+    }
+    mine = np;
+    if (np > 0) {
+      // mk_psources_maps: Delta z_RSD of the cell (grid_rvel after get_HI) and the cell's corner
+      CellState s;
+      cell_velocity<T>(m, F, ix, iy, iz, s);
+      const double dz_rsd = m.rsd ? s.vr * vgrowth_of_r(m.tab, s.r) : 0.0;
+      const double x0 = m.dx * ix - m.obs[0], y0 = m.dx * iy - m.obs[1], z0 = m.dx * iz - m.obs[2];
+      unsigned long long ck = ~0ull;
+      double ca = 0.0, cb = 0.0;
+      for (uint32_t src = 0; src < np; src++) {
+        double ux, uy, uz, unused;
+        ps_uniforms(ps.seed, cell, 4 * src + 1, 0, ux, uy);
+        ps_uniforms(ps.seed, cell, 4 * src + 1, 1, uz, unused);
+        const double px = x0 + m.dx * ux, py = y0 + m.dx * uy, pz = z0 + m.dx * uz;
+        const double r = sqrt(px * px + py * py + pz * pz);
+        const double red_cosmo = z_of_r(m.tab, r);
+        const double opz = 1.0 + red_cosmo + dz_rsd;
+        const int64_t ipix = vec2pix_ring(m.nside, px, py, pz, r);
+        double maxpdf, c, L = 0;
+        if (!ps_lum_setup(ps, red_cosmo, maxpdf, c) || (uint64_t)ipix >= (uint64_t)m.npix) continue;   // L = 0
+        for (uint32_t att = 0; att < PS_MAX_TRIES; att++)
+          if (ps_lum_try(ps.seed, cell, src, att, maxpdf, c, L)) break;
+        const double wgt = L / (r * r * opz);
+        const double a = wgt * pow(opz * 1e-3, -0.1), b = 10.0 * wgt * pow(opz * 1e-3, -0.75);
+        if ((unsigned long long)ipix == ck) {
+          ca += a;
+          cb += b;
+        }
+        else {
+          if (ck != ~0ull) ps_agg_add<SLOTS>(ck, ca, cb, keys, va, vb, ps.A, ps.B);
+          ck = (unsigned long long)ipix;
+          ca = a;
+          cb = b;
+        }
+      }
+      if (ck != ~0ull) ps_agg_add<SLOTS>(ck, ca, cb, keys, va, vb, ps.A, ps.B);
+    }
+  }
+  red[t] = mine;
+  __syncthreads();
+  for (int i = t; i < SLOTS; i += 256) {
+    const unsigned long long k = keys[i];
+    if (k != ~0ull) {
+      unsafeAtomicAdd(ps.A + k, va[i]);
+      unsafeAtomicAdd(ps.B + k, vb[i]);
+    }
+  }
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0 && red[0]) atomicAdd(ps.count, red[0]);
+}
+
+// maps_PS[inu][pix] = c1[inu] A[pix] + c2[inu] B[pix]
+template <typename T>
+__global__ __launch_bounds__(256) void k_ps_expand(const double *__restrict__ A, const double *__restrict__ B,
+                                                   const double *__restrict__ c1, const double *__restrict__ c2,
+                                                   int n_nu, int64_t npix, T *__restrict__ maps)
+{
+  const int inu = blockIdx.y;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < npix; p += (int64_t)gridDim.x * 256)
+    maps[(size_t)inu * npix + p] = (T)(c1[inu] * A[p] + c2[inu] * B[p]);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero(T *__restrict__ p, size_t n)
 {
@@ -305,6 +536,26 @@ hipError_t launch_grid_set(cx<T> *F, size_t ncell, int which, const double *in, 
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_psources(const cx<T> *F, const MapParams &m, const PsParams &ps, hipStream_t s)
+{
+  size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
+  unsigned gx = (unsigned)(ntiles < 65536 ? ntiles : 65536);
+  unsigned gy = (unsigned)((ntiles + gx - 1) / gx);
+  hipLaunchKernelGGL(k_psources<T>, dim3(gx, gy), dim3(256), 0, s, F, m, ps);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ps_expand(const double *A, const double *B, const double *c1, const double *c2, int n_nu,
+                            int64_t npix, T *maps, hipStream_t s)
+{
+  int64_t b = (npix + 255) / 256;
+  hipLaunchKernelGGL(k_ps_expand<T>, dim3((unsigned)(b < 4096 ? b : 4096), n_nu), dim3(256), 0, s, A, B, c1, c2,
+                     n_nu, npix, maps);
+  return hipGetLastError();
+}
+
 #define CRIME_INST(T)                                                                                   \
   template hipError_t launch_maps<T>(const cx<T> *, T *, size_t, const MapParams &, hipStream_t,      \
                                      hipEvent_t *, const Schedule &);                                   \
@@ -312,7 +563,10 @@ hipError_t launch_grid_set(cx<T> *F, size_t ncell, int which, const double *in, 
   template hipError_t launch_sample_indices<T>(const cx<T> *, const MapParams &, int64_t, int64_t,     \
                                                int32_t *, int64_t *, hipStream_t);                      \
   template hipError_t launch_grid_get<T>(const cx<T> *, size_t, int, double *, hipStream_t);          \
-  template hipError_t launch_grid_set<T>(cx<T> *, size_t, int, const double *, hipStream_t);
+  template hipError_t launch_grid_set<T>(cx<T> *, size_t, int, const double *, hipStream_t);          \
+  template hipError_t launch_psources<T>(const cx<T> *, const MapParams &, const PsParams &, hipStream_t); \
+  template hipError_t launch_ps_expand<T>(const double *, const double *, const double *, const double *, int, \
+                                          int64_t, T *, hipStream_t);
 CRIME_INST(float)
 CRIME_INST(double)
 #undef CRIME_INST

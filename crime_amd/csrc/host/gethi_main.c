@@ -98,6 +98,41 @@ int main(int argc, char **argv)
     printf(">    Relative time ellapsed %.1lf ms\n", wall_ms() - t0);
     free(maps);
   }
+  if (par.do_psources) {
+    /* src/main_gh.c:55-66 and the <prefix>_ps_###.fits files of src/io_gh.c:122-127 */
+    crime_gh_ps_stats ps;
+    printf("*** Getting point sources, making source maps\n");
+    rc = crime_gh_mk_psources_maps(h, &ps);
+    if (rc) goto fail;
+    printf("  There will be %lld particles in total \n", (long long)ps.n_sources);
+    printf(">    Relative time ellapsed %.1lf ms\n", ps.ms_sources + ps.ms_maps);
+    size_t npix = (size_t)12 * par.n_side * par.n_side, nmaps = (size_t)par.n_nu * npix;
+    double *ps64 = (double *)malloc(nmaps * sizeof(double));
+    float *ps32 = (float *)malloc(nmaps * sizeof(float));
+    if (!ps64 || !ps32 || crime_gh_copy_ps_maps(h, ps64)) {
+      free(ps64);
+      free(ps32);
+      fprintf(stderr, "CRIME: could not fetch the point-source maps\n");
+      crime_gh_destroy(h);
+      gh_params_free(&par);
+      return 1;
+    }
+    for (size_t i = 0; i < nmaps; i++) ps32[i] = (float)ps64[i];
+    printf("*** Writing single slice files %s_ps_###.fits\n", par.prefix_out);
+    for (int inu = 0; inu < par.n_nu && !rc; inu++) {
+      char fname[600];
+      snprintf(fname, sizeof(fname), "%s_ps_%03d.fits", par.prefix_out, inu + 1);
+      rc = gh_write_healpix_fits(fname, ps32 + npix * inu, par.n_side);
+    }
+    free(ps64);
+    free(ps32);
+    if (rc) {
+      fprintf(stderr, "CRIME: could not write the point-source maps\n");
+      crime_gh_destroy(h);
+      gh_params_free(&par);
+      return 1;
+    }
+  }
   printf(">    Total time ellapsed %.1lf ms\n", wall_ms() - t_all);
   printf("\n|-------------------------------------------------|\n\n");
   crime_gh_destroy(h);

@@ -131,6 +131,13 @@ enum {
   CRIME_GH_GRID_DZRSD = 4   /* RSD redshift shift (grid_rvel after grid_tools.c:146), read only */
 };
 
+/* point-source pass: timings (hipEvent, ms) and the number of sources drawn */
+typedef struct crime_gh_ps_stats {
+  double ms_sources;          /* Poisson counts, positions, luminosities, per-pixel sums */
+  double ms_maps;             /* the per-channel expansion into the point-source maps */
+  int64_t n_sources;
+} crime_gh_ps_stats;
+
 /* sizeof(crime_gh_desc) as compiled into the library, for FFI bindings to check their layout. */
 size_t crime_gh_desc_size(void);
 
@@ -200,6 +207,30 @@ int crime_gh_autotune(crime_gh_t *h, int reps, int *sched_out, double *ms_out);
 /* Device pointer and size of the resident maps (n_nu*12*Nside^2 elements of the grid precision),
  * for callers that hand the maps to further device work without a host round trip. */
 int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes);
+
+/* Point sources (do_psources, src/main_gh.c:55-66): setup_psources (src/psources.c:112-145),
+ * get_point_sources (src/grid_tools.c:24-101) and mk_psources_maps (src/pixelize.c:58-148) on the
+ * resident fields, into point-source maps of their own (maps_PS, written as <prefix>_ps_###.fits by
+ * src/io_gh.c:122-127), layout maps_PS[ipix + N_pix * inu], mK, the grid's precision.
+ *  - Per cell a Poisson count with mean n(z) dx^3 exp(g (delta - g sigma^2 / 2)), g = D(r) (bias 1),
+ *    at the cell centre offset as the reference offsets it (x by pos_obs[1], grid_tools.c:66).
+ *  - Per source a uniform position in the cell, its pixel, z + Delta z_RSD of the cell, and a
+ *    luminosity from the reference's rejection sampler; its temperature enters every channel.
+ *  - The SED is separable, so each source adds two terms to its pixel and the maps are expanded per
+ *    channel afterwards; the sum is the reference's up to rounding.
+ *  - Random numbers: Philox-4x32-10 keyed by (seed, 0x50535243) and counted by (cell, purpose, draw);
+ *    the reference's per-thread GSL streams depend on its OpenMP thread count.
+ * Requires crime_gh_create_d_and_vr_fields; may run before or after crime_gh_mk_T_maps. */
+int crime_gh_mk_psources_maps(crime_gh_t *h, crime_gh_ps_stats *st);
+
+/* Host copy of the point-source maps as FP64 (n_nu * N_pix values). */
+int crime_gh_copy_ps_maps(crime_gh_t *h, double *out);
+
+/* Device pointer and size of the point-source maps (NULL / 0 before the first pass). */
+int crime_gh_device_ps_maps(crime_gh_t *h, void **dptr, size_t *bytes);
+
+/* The two per-pixel sums A, B (N_pix FP64 each) of the last pass: T(pix, nu) = c1(nu) A + c2(nu) B. */
+int crime_gh_ps_pixel_sums(crime_gh_t *h, double *A, double *B);
 
 /* Batched runs (no counterpart in the reference, which runs one realisation per process).
  * A batch captures every member's field and map stages into ONE hipGraph, members forked over

@@ -38,7 +38,7 @@ typedef struct gh_params {
   int n_grid;
   long n_side;
   uint32_t seed;
-  int do_psources;          /* parsed for compatibility; point sources are not produced */
+  int do_psources;          /* 1: point-source maps <prefix>_ps_###.fits (crime_gh_mk_psources_maps) */
   /* ---- extension keys (ignored with a warning by the reference, src/io_gh.c:250-251) ---- */
   double l_box_override;    /* l_box=    (<=0: derived, src/cosmo.c:361) */
   int bias_model;           /* bias_model= reference|const|linear */

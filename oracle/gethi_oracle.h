@@ -37,6 +37,15 @@ int64_t orc_alm_index(int l, int m, int lmax);
 int orc_map2alm(int64_t nside, int lmax, int nmaps, const double *maps, double *alms);
 void orc_alm2cl(int lmax, int nmaps, const double *alms, double *cls);
 int orc_alm2map(int64_t nside, int lmax, int nmaps, const double *alms, double *maps);
+double orc_ps_lfunc(double L, double z);
+double orc_ps_sed(double nu);
+double orc_ps_temp(double hhub, double L, double nu_obs, double z, double r, double domega);
+void orc_ps_tables(double z_max, double *nz, double *maxpdf);
+uint32_t orc_ps_poisson(double lam, uint32_t seed, uint64_t cell);
+double orc_ps_luminosity(const double *nz, const double *maxpdf, double z_max, double z, uint32_t seed,
+                         uint64_t cell, uint32_t src);
+int64_t orc_psources(const crime_gh_desc *d, const double *dens, const double *dzrsd, double sigma2, double *maps,
+                     double *A, double *B);
 void orc_beam_window(int lmax, double fwhm_arcmin, double *out);
 void orc_add_noise(int64_t npix, int nmaps, const double *sigma2, uint32_t seed, double *maps);
 #endif

@@ -62,6 +62,19 @@ def lib():
         L.orc_map2alm.argtypes = [I64, C.c_int, C.c_int, _dp, _dp]
         L.orc_map2alm.restype = C.c_int
         L.orc_alm2cl.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_ps_lfunc.argtypes = [C.c_double, C.c_double]
+        L.orc_ps_lfunc.restype = C.c_double
+        L.orc_ps_sed.argtypes = [C.c_double]
+        L.orc_ps_sed.restype = C.c_double
+        L.orc_ps_temp.argtypes = [C.c_double] * 6
+        L.orc_ps_temp.restype = C.c_double
+        L.orc_ps_tables.argtypes = [C.c_double, _dp, _dp]
+        L.orc_ps_poisson.argtypes = [C.c_double, C.c_uint32, C.c_uint64]
+        L.orc_ps_poisson.restype = C.c_uint32
+        L.orc_ps_luminosity.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_uint32, C.c_uint64, C.c_uint32]
+        L.orc_ps_luminosity.restype = C.c_double
+        L.orc_psources.argtypes = [D, _dp, _dp, C.c_double, _dp, _dp, _dp]
+        L.orc_psources.restype = C.c_int64
         L.orc_alm2map.argtypes = [I64, C.c_int, C.c_int, _dp, _dp]
         L.orc_alm2map.restype = C.c_int
         L.orc_beam_window.argtypes = [C.c_int, C.c_double, _dp]
@@ -250,3 +263,33 @@ def add_noise(maps, sigma2, seed):
     s2 = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma2, np.float64), (m.shape[0],)))
     lib().orc_add_noise(m.shape[1], m.shape[0], _p(s2), seed, _p(m))
     return m
+
+
+PS_NZ = 256
+
+
+def ps_tables(z_max):
+    """setup_psources: n(z_i) and max_Lpdf at z_i = i z_max / 256."""
+    nz, mp = np.empty(PS_NZ), np.empty(PS_NZ)
+    lib().orc_ps_tables(z_max, _p(nz), _p(mp))
+    return nz, mp
+
+
+def ps_poisson(lam, seed, cells):
+    return np.array([lib().orc_ps_poisson(float(lam), seed, int(c)) for c in cells], dtype=np.int64)
+
+
+def ps_luminosity(z, z_max, seed, cells, src=0):
+    nz, mp = ps_tables(z_max)
+    return np.array([lib().orc_ps_luminosity(_p(nz), _p(mp), z_max, float(z), seed, int(c), src) for c in cells])
+
+
+def psources(desc, dens, dzrsd, sigma2, with_maps=True):
+    """Point sources of the grids: (maps (n_nu, Npix) or None, A, B, number of sources)."""
+    npix = 12 * desc.n_side ** 2
+    maps = np.zeros((desc.n_nu, npix)) if with_maps else None
+    a, b = np.zeros(npix), np.zeros(npix)
+    n = lib().orc_psources(C.byref(desc), _p(np.ascontiguousarray(dens, np.float64)),
+                           _p(np.ascontiguousarray(dzrsd, np.float64)), sigma2,
+                           _p(maps) if with_maps else None, _p(a), _p(b))
+    return maps, a, b, n
