@@ -115,21 +115,36 @@ def bench_ensemble(args):
         ms = sum(ts) / len(ts)
         print(f"bench: batched graph {ms:.1f} ms per ensemble", file=sys.stderr, flush=True)
         b.close()
-        # per-member cross-frequency C_l (l <= 3 Nside - 1) of the resident maps, one SHT plan per Nside
-        plans = {ns: crime_amd.Sht(ns, 3 * ns - 1, max_maps=64) for ns in sorted({h.n_side for h in hs})}
-        for h in hs[:2]:
-            plans[h.n_side].cls_of(h, cross=True)
-        # spectra stay in HBM like the maps; stage times summed from the plans' own events
-        t0 = time.perf_counter()
-        stage_ms = {}
+        # per-member cross-frequency C_l (l <= 3 Nside - 1) of the resident maps, left in HBM like the
+        # maps.  Members of one Nside are analysed cls_batch at a time in one call (their 64 x 64
+        # spectra each); the one-member-per-call path is timed beside it.  Stage times are summed
+        # from the plans' own events.
+        groups = {}
         for h in hs:
-            plans[h.n_side].cls_of(h, cross=True, to_host=False)
-            for k, v in plans[h.n_side].stats().items():
-                stage_ms[k] = stage_ms.get(k, 0.0) + v
-        ms_cls = (time.perf_counter() - t0) * 1e3
+            groups.setdefault(h.n_side, []).append(h)
+        nb = max(1, args.cls_batch)
+
+        def spectra(plans, batch):
+            stage, t0 = {}, time.perf_counter()
+            for ns, g in groups.items():
+                for i in range(0, len(g), batch):
+                    plans[ns].cls_of_members(g[i:i + batch], to_host=False)
+                    for k, v in plans[ns].stats().items():
+                        stage[k] = stage.get(k, 0.0) + v
+            return (time.perf_counter() - t0) * 1e3, stage
+
+        plans1 = {ns: crime_amd.Sht(ns, 3 * ns - 1, max_maps=64) for ns in groups}
+        spectra(plans1, 1)
+        ms_cls1, stage1 = spectra(plans1, 1)
+        for p in plans1.values():
+            p.close()
+        plans = {ns: crime_amd.Sht(ns, 3 * ns - 1, max_maps=64 * nb) for ns in groups}
+        spectra(plans, nb)
+        ms_cls, stage_ms = spectra(plans, nb)
         for p in plans.values():
             p.close()
-        print(f"bench: per-member cross C_l of {nmem} members in {ms_cls:.1f} ms", file=sys.stderr, flush=True)
+        print(f"bench: per-member cross C_l of {nmem} members in {ms_cls:.1f} ms ({nb} members per call), "
+              f"{ms_cls1:.1f} ms one per call", file=sys.stderr, flush=True)
         # the drop-in sequence for one mock (what one GetHI run per mock does): host set-up, device
         # set-up, run, maps to the host, release -- timed on 8 members of each kind
         sample = [i for k in range(3) for i in range(32 * k, 32 * k + 8)]
@@ -151,7 +166,8 @@ def bench_ensemble(args):
         "data": "synthetic (Philox modes of BBKS P(k) per member cosmology; resident in HBM)",
         "config": {"workload": "ensemble_96", "members": "32x128^3/Nside128 + 32x192^3/Nside256 + "
                    "32x256^3/Nside256", "n_nu": 64, "cells_per_ensemble": cells, "streams": args.streams},
-        "ms_cls_per_ensemble": ms_cls, "cls_stage_ms": stage_ms,
+        "ms_cls_per_ensemble": ms_cls, "cls_stage_ms": stage_ms, "cls_members_per_call": nb,
+        "ms_cls_one_member_per_call": ms_cls1, "cls_stage_ms_one_member_per_call": stage1,
         "mocks_per_hour_maps_and_cls": nmem * 3600e3 / (ms + ms_cls),
         "plain_resident_mocks_per_hour": nmem * 3600e3 / ms_plain,
         "plain_end_to_end_mocks_per_hour": 3600e3 / ms_e2e,
@@ -273,6 +289,7 @@ def main():
     ap.add_argument("--emu-steps", type=int, default=20000, help="training steps of the emulator workload")
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel schedule (no autotune)")
     ap.add_argument("--streams", type=int, default=4, help="concurrent streams of the ensemble graph")
+    ap.add_argument("--cls-batch", type=int, default=16, help="ensemble members per spectra call")
     ap.add_argument("--cpu-n", type=int, default=768, help="grid of the CPU-baseline sample (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (scale reference for PMC byte counters)")

@@ -375,6 +375,28 @@ class Sht:
                                                        _dptr(out)))
         return out
 
+    def cls_sets_device(self, dptrs, maps_per_set, fp64, to_host=True):
+        """Cross spectra within each set of maps_per_set device maps (set s at dptrs[s]):
+        (nsets, maps_per_set, maps_per_set, lmax+1), or (device pointer, bytes) with to_host=False."""
+        n = len(dptrs)
+        tab = (C.c_void_p * n)(*dptrs)
+        if not to_host:
+            self._check(self._lib.crime_sht_anafast_sets_device(self._p, n, tab, maps_per_set, int(fp64), None))
+            p, nb = C.c_void_p(), C.c_size_t()
+            self._check(self._lib.crime_sht_device_cls(self._p, C.byref(p), C.byref(nb)))
+            return p.value, nb.value
+        out = np.empty((n, maps_per_set, maps_per_set, self.lmax + 1))
+        self._check(self._lib.crime_sht_anafast_sets_device(self._p, n, tab, maps_per_set, int(fp64), _dptr(out)))
+        return out
+
+    def cls_of_members(self, ghs, to_host=True):
+        """Per-member cross spectra of several GetHI handles of this plan's Nside and one precision and
+        channel count, in one call."""
+        fp64 = {gh.precision == FP64 for gh in ghs}
+        nnu = {gh.n_nu for gh in ghs}
+        assert len(fp64) == 1 and len(nnu) == 1
+        return self.cls_sets_device([gh.device_maps()[0] for gh in ghs], nnu.pop(), fp64.pop(), to_host)
+
     def cls_pairs_device(self, dptr, nmaps, fp64, pairs):
         """C_l of the listed (a, b) map pairs of nmaps device maps: (npairs, lmax+1)."""
         pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))

@@ -203,6 +203,8 @@ def gpu_lib():
         lib.crime_sht_anafast.argtypes = [H, C.c_int, _dp, C.c_int, _dp]
         lib.crime_sht_device_cls.restype = C.c_int
         lib.crime_sht_device_cls.argtypes = [H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
+        lib.crime_sht_anafast_sets_device.restype = C.c_int
+        lib.crime_sht_anafast_sets_device.argtypes = [H, C.c_int, C.POINTER(C.c_void_p), C.c_int, C.c_int, _dp]
         lib.crime_sht_alm2map.restype = C.c_int
         lib.crime_sht_alm2map.argtypes = [H, C.c_int, _dp, _dp]
         lib.crime_sht_alm2map_device.restype = C.c_int

@@ -46,6 +46,8 @@ struct crime_sht {
   int4 *d_tiles = nullptr, *d_syn_tiles = nullptr;
   double *d_sig2 = nullptr;     // per-map beam sigma^2 or noise variance
   uint8_t *d_mask = nullptr;    // Npix, allocated by the first mask call
+  const void **d_mtab = nullptr;          // device table of map-set base pointers (max_maps entries)
+  std::vector<const void *> h_mtab;       // its host image (kept for the asynchronous upload)
   bool syn_ok = true;           // every split-DFT ring is shorter than 32 pixels
   double *d_G = nullptr, *d_alm = nullptr, *d_lam = nullptr;
   double *d_cls = nullptr;
@@ -94,7 +96,7 @@ extern "C" void crime_sht_destroy(crime_sht_t *p)
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   void *bufs[] = {p->d_rings, p->d_z, p->d_st, p->d_coef, p->d_off, p->d_tiles, p->d_syn_tiles, p->d_sig2,
-                  p->d_mask, p->d_G,
+                  p->d_mask, (void *)p->d_mtab, p->d_G,
                   p->d_alm, p->d_lam, p->d_cls, p->d_unpack, p->d_hmaps, p->d_tw,
                   p->d_woff, p->d_boff, p->d_wtab, p->d_btab};
   for (void *b : bufs)
@@ -341,6 +343,8 @@ static int create_impl(crime_sht *p, int64_t nside, int lmax, int max_maps, int 
   SHT_HIP(p, hipMalloc((void **)&p->d_syn_tiles, syn_tiles.size() * sizeof(int4)));
   SHT_HIP(p, hipMemcpy(p->d_syn_tiles, syn_tiles.data(), syn_tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
   SHT_HIP(p, hipMalloc((void **)&p->d_sig2, (size_t)max_maps * sizeof(double)));
+  SHT_HIP(p, hipMalloc((void **)&p->d_mtab, (size_t)max_maps * sizeof(void *)));
+  p->h_mtab.resize(max_maps);
   SHT_HIP(p, hipMalloc((void **)&p->d_lam, max_chunk_rows * row_bytes));
 
   const size_t gsz = (size_t)(lmax + 1) * 2 * p->RP * p->ldg * sizeof(double);
@@ -400,14 +404,26 @@ static ShtLaunch ring_launch(const crime_sht *p, ShtLaunch g, int nmaps)
   return g;
 }
 
-static int analyse(crime_sht *p, int nmaps, const void *d_maps, int fp64)
+// ring spectra + alm GEMMs of nsets sets of mps device maps each (set s at d_maps[s]) into p->d_alm,
+// columns 2 (s mps + t) + (re | im); stage times into p->st
+static int analyse_sets(crime_sht *p, int nsets, const void *const *d_maps, int mps, int fp64)
 {
-  if (nmaps < 1 || nmaps > p->max_maps)
-    return sht_error(p, CRIME_GH_EINVAL, "nmaps %d outside [1, %d]", nmaps, p->max_maps);
+  if (nsets < 1 || mps < 1 || (int64_t)nsets * mps > p->max_maps)
+    return sht_error(p, CRIME_GH_EINVAL, "%d sets of %d maps: outside [1, %d] maps", nsets, mps, p->max_maps);
   if (!d_maps) return sht_error(p, CRIME_GH_EINVAL, "maps are NULL");
+  for (int i = 0; i < nsets; i++) {
+    if (!d_maps[i]) return sht_error(p, CRIME_GH_EINVAL, "maps of set %d are NULL", i);
+    p->h_mtab[i] = d_maps[i];
+  }
+  const int nmaps = nsets * mps;
   hipEvent_t *ev = p->ev.data();
+  SHT_HIP(p, hipMemcpyAsync(p->d_mtab, p->h_mtab.data(), nsets * sizeof(void *), hipMemcpyHostToDevice, p->stream));
   SHT_HIP(p, hipEventRecord(ev[0], p->stream));
-  for (const ShtLaunch &g : p->groups) SHT_HIP(p, launch_rings(ring_launch(p, g, nmaps), d_maps, fp64));
+  for (const ShtLaunch &g : p->groups) {
+    ShtLaunch a = ring_launch(p, g, nmaps);
+    a.mps = mps;
+    SHT_HIP(p, launch_rings(a, (const void *const *)p->d_mtab, fp64));
+  }
   SHT_HIP(p, hipEventRecord(ev[1], p->stream));
   for (size_t i = 0; i < p->chunks.size(); i++) {
     const Chunk &c = p->chunks[i];
@@ -420,6 +436,11 @@ static int analyse(crime_sht *p, int nmaps, const void *d_maps, int fp64)
     SHT_HIP(p, hipEventRecord(ev[3 + 2 * i], p->stream));
   }
   return 0;
+}
+
+static int analyse(crime_sht *p, int nmaps, const void *d_maps, int fp64)
+{
+  return analyse_sets(p, 1, &d_maps, nmaps, fp64);
 }
 
 static int finish(crime_sht *p, bool with_cls)
@@ -500,6 +521,29 @@ extern "C" int crime_sht_anafast_device(crime_sht_t *p, int nmaps, const void *d
     p->cls_cap = n;
   }
   SHT_HIP(p, launch_cls(p->d_alm, p->lmax, nmaps, p->ldo, cross, p->d_cls, p->stream));
+  rc = finish(p, true);
+  if (rc) return rc;
+  p->cls_bytes = n * sizeof(double);
+  if (cls) SHT_HIP(p, hipMemcpy(cls, p->d_cls, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int crime_sht_anafast_sets_device(crime_sht_t *p, int nsets, const void *const *d_maps, int maps_per_set,
+                                             int maps_fp64, double *cls)
+{
+  if (!p) return sht_error(nullptr, CRIME_GH_EINVAL, "plan is NULL");
+  SHT_HIP(p, hipSetDevice(p->device));
+  int rc = analyse_sets(p, nsets, d_maps, maps_per_set, maps_fp64);
+  if (rc) return rc;
+  const size_t n = (size_t)nsets * maps_per_set * maps_per_set * (p->lmax + 1);
+  if (n > p->cls_cap) {
+    if (p->d_cls) SHT_HIP(p, hipFree(p->d_cls));
+    p->d_cls = nullptr;
+    p->cls_cap = 0;
+    SHT_HIP(p, hipMalloc((void **)&p->d_cls, n * sizeof(double)));
+    p->cls_cap = n;
+  }
+  SHT_HIP(p, launch_cls_sets(p->d_alm, p->lmax, nsets, maps_per_set, p->ldo, p->d_cls, p->stream));
   rc = finish(p, true);
   if (rc) return rc;
   p->cls_bytes = n * sizeof(double);

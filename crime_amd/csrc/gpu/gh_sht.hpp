@@ -41,9 +41,11 @@ struct ShtLaunch {
   const void *wtab;
   const int64_t *boff;
   const void *B;
+  int mps;                       // maps per entry of the map table (analysis)
   hipStream_t s;
 };
-hipError_t launch_rings(const ShtLaunch &a, const void *maps, int fp64);
+// analysis: map c of the nmaps is mtab[c / mps] + (c % mps) Npix (mtab: device table of map sets)
+hipError_t launch_rings(const ShtLaunch &a, const void *const *mtab, int fp64);
 // synthesis: the same ring groups, from G into FP32 or FP64 maps
 hipError_t launch_syn_rings(const ShtLaunch &a, void *maps, int fp64);
 // B of the Bluestein rings [r0, r0 + nr) of one convolution length M
@@ -59,6 +61,8 @@ hipError_t launch_legendre(const double2 *coef, const int64_t *off, int m0, int 
 constexpr int ALM_ROWS = CRIME_ALM_ROWS;
 hipError_t launch_alm_gemm(const double *lam, const double *G, const int4 *tiles, int ntiles, const int64_t *off,
                            int m0, int lmax, int RP, int ldg, int ncols, double *alm, int ldo, hipStream_t s);
+// cross spectra within each of nsets consecutive sets of mps maps: cls[set][a][b][l]
+hipError_t launch_cls_sets(const double *alm, int lmax, int nsets, int mps, int ldo, double *cls, hipStream_t s);
 hipError_t launch_cls(const double *alm, int lmax, int nmaps, int ldo, int cross, double *cls, hipStream_t s);
 hipError_t launch_cls_pairs(const double *alm, int lmax, int ldo, int npairs, const int *pairs, double *cls,
                             hipStream_t s);

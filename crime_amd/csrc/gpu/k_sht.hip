@@ -85,7 +85,7 @@ __device__ void ring_spectrum(const TM *__restrict__ f_g, int n, int K, int P, c
 }
 
 template <typename TM>
-__global__ __launch_bounds__(512) void k_ring_dft(const TM *__restrict__ maps, int64_t npix,
+__global__ __launch_bounds__(512) void k_ring_dft(const TM *const *__restrict__ mtab, int mps, int64_t npix,
                                                   const ShtRing *__restrict__ rings, int R, int RP, int lmax,
                                                   int ldg, double w, double *__restrict__ G, int r0)
 {
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(512) void k_ring_dft(const TM *__restrict__ maps, i
     sincospi(-2.0 * (double)t / (double)n, &sn, &cs);
     tw[t] = make_double2(cs, sn);
   }
-  const TM *mp = maps + (size_t)c * npix;
+  const TM *mp = mtab[c / mps] + (size_t)(c % mps) * npix;
   ring_spectrum(mp + ri.first_n, n, K, P, tw, f, S, Dn);
   const bool eq = (r == R - 1);
   if (!eq) ring_spectrum(mp + ri.first_s, n, K, P, tw, f, S, Ds);
@@ -162,7 +162,8 @@ struct RingFftCfg {
 };
 
 template <typename TM, int L>
-__global__ __launch_bounds__(RingFftCfg<L>::NT) void k_ring_fft(const TM *__restrict__ maps, int64_t npix,
+__global__ __launch_bounds__(RingFftCfg<L>::NT) void k_ring_fft(const TM *const *__restrict__ mtab, int mps,
+                                                               int64_t npix,
                                                                const ShtRing *__restrict__ rings, int R, int RP,
                                                                int lmax, int ldg, double w, double *__restrict__ G,
                                                                int r0, int nmaps, int nlines,
@@ -176,8 +177,9 @@ __global__ __launch_bounds__(RingFftCfg<L>::NT) void k_ring_fft(const TM *__rest
   const int c = live ? line % nmaps : 0, r = r0 + (live ? line / nmaps : 0);
   const ShtRing ri = rings[r];
   const bool eq = (r == R - 1);
-  const TM *fn = maps + (size_t)c * npix + ri.first_n;
-  const TM *fs = maps + (size_t)c * npix + ri.first_s;
+  const TM *base = mtab[c / mps] + (size_t)(c % mps) * npix;   // map c of set c / mps
+  const TM *fn = base + ri.first_n;
+  const TM *fs = base + ri.first_s;
   cx<double> *buf = lds + g * lpadded(L);
   auto load = [&](int e) -> cx<double> {
     if (!live) return {0.0, 0.0};
@@ -255,7 +257,8 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_blu_setup(const ShtRing *__re
 }
 
 template <typename TM, int M>
-__global__ __launch_bounds__(BluCfg<M>::NT) void k_ring_blu(const TM *__restrict__ maps, int64_t npix,
+__global__ __launch_bounds__(BluCfg<M>::NT) void k_ring_blu(const TM *const *__restrict__ mtab, int mps,
+                                                           int64_t npix,
                                                            const ShtRing *__restrict__ rings, int R, int RP, int lmax,
                                                            int ldg, double wn, double *__restrict__ G, int r0,
                                                            int nmaps, int nlines, const int64_t *__restrict__ woff,
@@ -273,8 +276,9 @@ __global__ __launch_bounds__(BluCfg<M>::NT) void k_ring_blu(const TM *__restrict
   const ShtRing ri = rings[r];
   const int n = ri.nphi;
   const bool eq = (r == R - 1);
-  const TM *fn = maps + (size_t)c * npix + ri.first_n;
-  const TM *fs = maps + (size_t)c * npix + ri.first_s;
+  const TM *base = mtab[c / mps] + (size_t)(c % mps) * npix;   // map c of set c / mps
+  const TM *fn = base + ri.first_n;
+  const TM *fs = base + ri.first_s;
   const cx<double> *w = wtab + woff[r];
   const cx<double> *Bk = B + boff[r];
   cx<double> *buf = lds + g * lpadded(M);
@@ -611,23 +615,27 @@ __global__ __launch_bounds__(256) void k_alm_gemm(const double *__restrict__ lam
 // C_l from alm[idx][2 t + (0 re | 1 im)]: cross spectra of all map pairs, autospectra, listed pairs.
 // ---------------------------------------------------------------------------------------------
 // Cross spectra on FP64 MFMA: for each l, C_l = X Y^T / (2l + 1) over k = 2m + (0 re | 1 im), with
-// X[a][k] = w_m alm_a, Y[b][k] = alm_b, w_0 = 1, w_m = 2.  One workgroup per (64 x 64 map tile with
-// ta <= tb, l); off-diagonal tiles also write the mirror block, so C^{ab} and C^{ba} are equal bit for
-// bit.  Consecutive blocks of one XCD take consecutive l of one tile, so the [a][b][l] stores of
+// X[a][k] = w_m alm_a, Y[b][k] = alm_b, w_0 = 1, w_m = 2, for the pairs within each of nsets sets of
+// mps maps (one set: every pair).  One workgroup per (set, 64 x 64 map tile with ta <= tb, l);
+// off-diagonal tiles also write the mirror block, so C^{ab} and C^{ba} are equal bit for bit.
+// Consecutive blocks of one XCD take consecutive l of one tile, so the [set][a][b][l] stores of
 // neighbouring l meet in that XCD's L2.
-__global__ __launch_bounds__(256) void k_cls_cross(const double *__restrict__ alm, int lmax, int nmaps, int ldo,
+__global__ __launch_bounds__(256) void k_cls_cross(const double *__restrict__ alm, int lmax, int mps, int ldo,
                                                    int ntile, double *__restrict__ cls)
 {
   const int logical = xcd_swizzle(blockIdx.x, gridDim.x);
   const int nl = lmax + 1;
   const int l = lmax - logical % nl;
-  int rem = logical / nl, ta = 0;
+  const int npair = ntile * (ntile + 1) / 2;
+  const int set = logical / nl / npair;
+  int rem = logical / nl % npair, ta = 0;
   while (rem >= ntile - ta) {
     rem -= ntile - ta;
     ta++;
   }
   const int tb = ta + rem;
-  const int a0 = ta * GT, b0 = tb * GT;
+  const int a0 = ta * GT, b0 = tb * GT;           // within the set
+  const int cbase = set * mps;                    // the set's first map in the alm columns
   __shared__ double As[GT][GK + 1];
   __shared__ double Bs[GK][GT + 2];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -641,12 +649,12 @@ __global__ __launch_bounds__(256) void k_cls_cross(const double *__restrict__ al
     double2 xa0 = make_double2(0, 0), xa1 = xa0, yb0 = xa0, yb1 = xa0;
     if (m <= l) {
       const int64_t idx = m > 0 ? l + (int64_t)m * lmax - (int64_t)m * (m - 1) / 2 : l;
-      const double *row = alm + (size_t)idx * ldo;
+      const double *row = alm + (size_t)idx * ldo + 2 * cbase;
       const double wgt = m ? 2.0 : 1.0;
-      if (a0 + q < nmaps) xa0 = *(const double2 *)(row + 2 * (a0 + q));
-      if (a0 + q + 1 < nmaps) xa1 = *(const double2 *)(row + 2 * (a0 + q + 1));
-      if (b0 + q < nmaps) yb0 = *(const double2 *)(row + 2 * (b0 + q));
-      if (b0 + q + 1 < nmaps) yb1 = *(const double2 *)(row + 2 * (b0 + q + 1));
+      if (a0 + q < mps) xa0 = *(const double2 *)(row + 2 * (a0 + q));
+      if (a0 + q + 1 < mps) xa1 = *(const double2 *)(row + 2 * (a0 + q + 1));
+      if (b0 + q < mps) yb0 = *(const double2 *)(row + 2 * (b0 + q));
+      if (b0 + q + 1 < mps) yb1 = *(const double2 *)(row + 2 * (b0 + q + 1));
       xa0.x *= wgt; xa0.y *= wgt; xa1.x *= wgt; xa1.y *= wgt;
     }
     As[q][2 * mi] = xa0.x;
@@ -669,15 +677,16 @@ __global__ __launch_bounds__(256) void k_cls_cross(const double *__restrict__ al
     __syncthreads();
   }
   const double inv = 1.0 / (2.0 * l + 1.0);
+  double *out = cls + (size_t)set * mps * mps * nl;
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++)
       for (int v = 0; v < 4; v++) {
         const int a = a0 + wr + 16 * i + (lane >> 4) + 4 * v;
         const int b = b0 + wc + 16 * j + (lane & 15);
-        if (a < nmaps && b < nmaps) {
+        if (a < mps && b < mps) {
           const double c = acc[i][j][v] * inv;
-          cls[((size_t)a * nmaps + b) * nl + l] = c;
-          if (ta != tb) cls[((size_t)b * nmaps + a) * nl + l] = c;
+          out[((size_t)a * mps + b) * nl + l] = c;
+          if (ta != tb) out[((size_t)b * mps + a) * nl + l] = c;
         }
       }
 }
@@ -891,20 +900,22 @@ hipError_t sht_set_lds_limits(size_t bytes)
 }
 
 template <typename TM, int L>
-static void launch_ring_fft_t(const ShtLaunch &a, const TM *maps)
+static void launch_ring_fft_t(const ShtLaunch &a, const TM *const *maps)
 {
   using C = RingFftCfg<L>;
   const int nlines = a.nmaps * a.nr;
-  hipLaunchKernelGGL((k_ring_fft<TM, L>), dim3((nlines + C::G - 1) / C::G), dim3(C::NT), 0, a.s, maps, a.npix, a.rings,
+  hipLaunchKernelGGL((k_ring_fft<TM, L>), dim3((nlines + C::G - 1) / C::G), dim3(C::NT), 0, a.s, maps, a.mps, a.npix,
+                     a.rings,
                      a.R, a.RP, a.lmax, a.ldg, a.w, a.G, a.r0, a.nmaps, nlines, (const cx<double> *)a.tw);
 }
 
 template <typename TM, int M>
-static void launch_ring_blu_t(const ShtLaunch &a, const TM *maps)
+static void launch_ring_blu_t(const ShtLaunch &a, const TM *const *maps)
 {
   using C = BluCfg<M>;
   const int nlines = a.nmaps * a.nr;
-  hipLaunchKernelGGL((k_ring_blu<TM, M>), dim3((nlines + C::G - 1) / C::G), dim3(C::NT), 0, a.s, maps, a.npix, a.rings,
+  hipLaunchKernelGGL((k_ring_blu<TM, M>), dim3((nlines + C::G - 1) / C::G), dim3(C::NT), 0, a.s, maps, a.mps, a.npix,
+                     a.rings,
                      a.R, a.RP, a.lmax, a.ldg, a.w, a.G, a.r0, a.nmaps, nlines, a.woff, (const cx<double> *)a.wtab,
                      a.boff, (const cx<double> *)a.B, (const cx<double> *)a.tw);
 }
@@ -965,11 +976,12 @@ hipError_t launch_syn_rings(const ShtLaunch &a, void *maps, int fp64)
 }
 
 template <typename TM>
-static hipError_t launch_rings_t(const ShtLaunch &a, const TM *maps)
+static hipError_t launch_rings_t(const ShtLaunch &a, const TM *const *maps)
 {
   switch (a.kind) {
   case SHT_RING_SPLIT:
-    hipLaunchKernelGGL(k_ring_dft<TM>, dim3(a.nmaps, a.nr), dim3(512), a.lds, a.s, maps, a.npix, a.rings, a.R, a.RP,
+    hipLaunchKernelGGL(k_ring_dft<TM>, dim3(a.nmaps, a.nr), dim3(512), a.lds, a.s, maps, a.mps, a.npix, a.rings, a.R,
+                       a.RP,
                        a.lmax, a.ldg, a.w, a.G, a.r0);
     break;
   case SHT_RING_FFT:
@@ -1006,9 +1018,10 @@ int ring_fft_length(int64_t nside)
   }
 }
 
-hipError_t launch_rings(const ShtLaunch &a, const void *maps, int fp64)
+hipError_t launch_rings(const ShtLaunch &a, const void *const *mtab, int fp64)
 {
-  return fp64 ? launch_rings_t<double>(a, (const double *)maps) : launch_rings_t<float>(a, (const float *)maps);
+  return fp64 ? launch_rings_t<double>(a, (const double *const *)mtab)
+              : launch_rings_t<float>(a, (const float *const *)mtab);
 }
 
 template <int M>
@@ -1052,13 +1065,19 @@ hipError_t launch_alm_gemm(const double *lam, const double *G, const int4 *tiles
   return hipGetLastError();
 }
 
+hipError_t launch_cls_sets(const double *alm, int lmax, int nsets, int mps, int ldo, double *cls, hipStream_t s)
+{
+  const int nt = (mps + GT - 1) / GT;
+  const int64_t nblk = (int64_t)nsets * nt * (nt + 1) / 2 * (lmax + 1);
+  if (nblk > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_cls_cross, dim3((unsigned)nblk), dim3(256), 0, s, alm, lmax, mps, ldo, nt, cls);
+  return hipGetLastError();
+}
+
 hipError_t launch_cls(const double *alm, int lmax, int nmaps, int ldo, int cross, double *cls, hipStream_t s)
 {
   if (cross) {
-    const int nt = (nmaps + GT - 1) / GT;
-    const int64_t nblk = (int64_t)nt * (nt + 1) / 2 * (lmax + 1);
-    if (nblk > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_cls_cross, dim3((unsigned)nblk), dim3(256), 0, s, alm, lmax, nmaps, ldo, nt, cls);
+    return launch_cls_sets(alm, lmax, 1, nmaps, ldo, cls, s);
   } else {
     dim3 grid((nmaps + 255) / 256, lmax + 1);
     hipLaunchKernelGGL(k_cls_auto, grid, dim3(256), 0, s, alm, lmax, nmaps, ldo, cls);

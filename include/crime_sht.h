@@ -54,6 +54,14 @@ int crime_sht_map2alm_device(crime_sht_t *p, int nmaps, const void *d_maps, int 
 int crime_sht_anafast_device(crime_sht_t *p, int nmaps, const void *d_maps, int maps_fp64, int cross,
                              double *cls);
 
+/* Cross spectra within each of nsets sets of maps_per_set device maps, in one call: set s has its
+ * maps at d_maps[s] (a host array of nsets device pointers, each to maps_per_set * Npix values),
+ * nsets * maps_per_set <= max_maps.  cls[((s * maps_per_set + a) * maps_per_set + b) * (lmax + 1) + l];
+ * NULL keeps them on the device (crime_sht_device_cls).  For an ensemble of mocks of one Nside: the
+ * per-member spectra of many members share the launches, GEMM tiles and LDS of one analysis. */
+int crime_sht_anafast_sets_device(crime_sht_t *p, int nsets, const void *const *d_maps, int maps_per_set,
+                                  int maps_fp64, double *cls);
+
 /* Device pointer and byte count of the last spectra computed (valid until the next call). */
 int crime_sht_device_cls(const crime_sht_t *p, const double **dptr, size_t *bytes);
 

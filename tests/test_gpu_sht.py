@@ -152,3 +152,21 @@ def test_production_maps_cross_spectra(cfgdir):
     prod = np.where(ms == 0, 1.0, 2.0) * (a * np.conj(b)).real
     ref_cl = np.bincount(ls, weights=prod, minlength=lmax + 1) / (2 * np.arange(lmax + 1) + 1)
     assert np.abs(cls[17, 42] - ref_cl).max() < 1e-10 * np.abs(ref_cl).max()
+
+
+def test_member_sets_equal_single_calls(cfgdir):
+    """Spectra of three members in one call (crime_sht_anafast_sets_device) equal each member's own."""
+    hs = [crime_amd.GetHI(small_params(cfgdir, f"set{i}", seed=500 + i, n_side=32, n_nu=8)) for i in range(3)]
+    try:
+        for h in hs:
+            h.run()
+        with crime_amd.Sht(32, 95, max_maps=24) as sh:
+            sets = sh.cls_of_members(hs)
+            single = [sh.cls_of(h, cross=True) for h in hs]
+    finally:
+        for h in hs:
+            h.close()
+    assert sets.shape == (3, 8, 8, 96)
+    for i in range(3):
+        assert np.allclose(sets[i], single[i], rtol=1e-13, atol=0)
+    assert not np.allclose(sets[0], sets[1])
