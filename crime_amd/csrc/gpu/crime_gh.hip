@@ -66,12 +66,14 @@ static int set_error(crime_gh *h, int code, const char *fmt, ...)
   return code;
 }
 
-#define HIP_OR_FAIL(h, call)                                                                        \
-  do {                                                                                              \
-    hipError_t e_ = (call);                                                                         \
-    if (e_ != hipSuccess)                                                                           \
-      return set_error(h, e_ == hipErrorOutOfMemory ? CRIME_GH_ENOMEM : CRIME_GH_EHIP, "%s: %s", #call, \
-                       hipGetErrorString(e_));                                                      \
+#define HIP_OR_FAIL(h, call)                                                                          \
+  do {                                                                                                \
+    hipError_t e_ = (call);                                                                           \
+    if (e_ != hipSuccess) {                                                                           \
+      (void)hipGetLastError(); /* reported here: a later launch check must not see it again */        \
+      return set_error(h, e_ == hipErrorOutOfMemory ? CRIME_GH_ENOMEM : CRIME_GH_EHIP, "%s: %s", #call,     \
+                       hipGetErrorString(e_));                                                        \
+    }                                                                                                 \
   } while (0)
 
 static bool grid_supported(int n)
@@ -921,7 +923,10 @@ static int batch_error(crime_gh_batch *b, int code, const char *fmt, ...)
 #define BATCH_HIP(b, call)                                                                           \
   do {                                                                                              \
     hipError_t e_ = (call);                                                                         \
-    if (e_ != hipSuccess) return batch_error(b, CRIME_GH_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    if (e_ != hipSuccess) {                                                                         \
+      (void)hipGetLastError(); /* reported here: a later launch check must not see it again */      \
+      return batch_error(b, CRIME_GH_EHIP, "%s: %s", #call, hipGetErrorString(e_));                 \
+    }                                                                                               \
   } while (0)
 
 static void batch_free(crime_gh_batch *b)

@@ -337,12 +337,14 @@ static int mlp_error(crime_mlp *h, int code, const char *fmt, ...)
   return code;
 }
 
-#define MLP_HIP(h, call)                                                                              \
+#define MLP_HIP(h, call)                                                                          \
   do {                                                                                                \
     hipError_t e_ = (call);                                                                           \
-    if (e_ != hipSuccess)                                                                             \
-      return mlp_error(h, e_ == hipErrorOutOfMemory ? CRIME_GH_ENOMEM : CRIME_GH_EHIP, "%s: %s", #call, \
+    if (e_ != hipSuccess) {                                                                           \
+      (void)hipGetLastError(); /* reported here: a later launch check must not see it again */        \
+      return mlp_error(h, e_ == hipErrorOutOfMemory ? CRIME_GH_ENOMEM : CRIME_GH_EHIP, "%s: %s", #call,     \
                        hipGetErrorString(e_));                                                        \
+    }                                                                                                 \
   } while (0)
 
 extern "C" int64_t crime_mlp_param_count(void) { return NPARAM; }

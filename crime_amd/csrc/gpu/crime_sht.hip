@@ -80,12 +80,14 @@ static int sht_error(crime_sht *p, int code, const char *fmt, ...)
   return code;
 }
 
-#define SHT_HIP(p, call)                                                                              \
+#define SHT_HIP(p, call)                                                                          \
   do {                                                                                                \
     hipError_t e_ = (call);                                                                           \
-    if (e_ != hipSuccess)                                                                             \
-      return sht_error(p, e_ == hipErrorOutOfMemory ? CRIME_GH_ENOMEM : CRIME_GH_EHIP, "%s: %s", #call, \
+    if (e_ != hipSuccess) {                                                                           \
+      (void)hipGetLastError(); /* reported here: a later launch check must not see it again */        \
+      return sht_error(p, e_ == hipErrorOutOfMemory ? CRIME_GH_ENOMEM : CRIME_GH_EHIP, "%s: %s", #call,     \
                        hipGetErrorString(e_));                                                        \
+    }                                                                                                 \
   } while (0)
 
 extern "C" const char *crime_sht_last_error(const crime_sht_t *p) { return p ? p->err.c_str() : g_sht_err.c_str(); }

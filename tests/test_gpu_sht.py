@@ -170,3 +170,18 @@ def test_member_sets_equal_single_calls(cfgdir):
     for i in range(3):
         assert np.allclose(sets[i], single[i], rtol=1e-13, atol=0)
     assert not np.allclose(sets[0], sets[1])
+
+
+def test_plan_beyond_device_memory_reports_enomem():
+    """A plan whose spectra buffer cannot fit (max_maps far beyond HBM) fails with ENOMEM and a message,
+    and the library stays usable."""
+    from crime_amd import _ffi
+    import ctypes as C
+
+    lib = _ffi.gpu_lib()
+    h = C.c_void_p()
+    rc = lib.crime_sht_create(512, 1535, 1 << 20, 0, C.byref(h))
+    assert rc == -2 and h.value is None
+    assert b"hipMalloc" in lib.crime_sht_last_error(None)
+    with crime_amd.Sht(8, 23, max_maps=1) as sh:
+        assert sh.map2alm(np.ones((1, 768))).shape == (1, 300)
