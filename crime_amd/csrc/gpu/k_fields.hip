@@ -156,7 +156,10 @@ __global__ __launch_bounds__((ColCfg<T, N, V>::NT)) void k_fft_y(cx<T> *__restri
   using C = ColCfg<T, N, V>;
   __shared__ cx<T> lds[N * C::TX];
   const int c = threadIdx.x % C::TX, t = threadIdx.x / C::TX;
-  const int kx = blockIdx.x * C::TX + c, kz = blockIdx.y;
+  // XCD-aware tile order: neighbouring column tiles of one plane run on one XCD, so tiles narrower
+  // than a 128-byte line share each line through that XCD's L2 instead of fetching it twice
+  const int b = xcd_swizzle((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
+  const int kx = (b % (int)gridDim.x) * C::TX + c, kz = b / (int)gridDim.x;
   cx<T> *col = F + (size_t)kx + (size_t)N * N * kz;
   auto load = [&](int e) { return col[(size_t)e * N]; };
   auto store = [&](int e, cx<T> v) { col[(size_t)e * N] = v; };
