@@ -101,9 +101,18 @@ def test_production_map_total(production):
 
 
 def test_largest_2048_runs(tmp_path):
+    """The 2048^3 mock (400 channels, Nside 1024, 4^3 sub-cells) and the cross spectra of its first 64
+    channels at lmax 3071 (Legendre table rebuilt in m-chunks): symmetric, non-negative autos."""
     par = crime_amd.Params(cf.largest_2048(str(tmp_path)))
     with crime_amd.GetHI(par) as gh:
         st = gh.run()
         ptr, nbytes = gh.device_maps()
+        with crime_amd.Sht(gh.n_side, 3 * gh.n_side - 1, max_maps=64) as sh:
+            cls = sh.cls_of(gh, cross=True, channels=64)
+            sst = sh.stats()
     print(f"2048^3: {st['ms_total']:.1f} ms, sigma2 {st['sigma2_gauss']:.6f}, maps {nbytes / 2 ** 30:.1f} GiB")
+    print(f"C_l of 64 channels at Nside 1024, lmax 3071: {sst}")
     assert np.isfinite(st["sigma2_gauss"]) and 0.05 < st["sigma2_gauss"] < 5
+    assert np.all(np.isfinite(cls)) and cls.shape == (64, 64, 3072)
+    assert np.allclose(cls, cls.transpose(1, 0, 2), rtol=1e-12, atol=0)
+    assert np.all(np.einsum("aal->al", cls) >= 0) and cls[0, 0, 2:].max() > 0
