@@ -28,7 +28,7 @@ struct crime_gh {
   double *d_pk = nullptr;      // logk | pk
   double *d_edges = nullptr;   // [n_nu+1]
   double *d_prefac = nullptr;  // [n_nu]
-  int *d_guess = nullptr;      // channel guess table
+  ChanRec *d_crec = nullptr;   // channel records of the frequency buckets
   void *d_amp = nullptr;       // amplitude per |k|^2
   uint32_t *d_order = nullptr; // map-stage tiles by distance from the observer
   double *d_sub = nullptr;     // [3*n_sub]
@@ -182,7 +182,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_guess, h->d_amp, h->d_order, h->d_sub,
+  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig, h->d_ps, h->d_ps_count, h->ps_maps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -300,8 +300,19 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     }
     guess[j] = lo;
   }
-  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_guess, n_guess * sizeof(int)));
-  HIP_OR_FAIL(h, hipMemcpy(h->d_guess, guess.data(), n_guess * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<ChanRec> crec(n_guess);
+  for (int j = 0; j < n_guess; j++) {
+    const int c = guess[j], last = c == d->n_nu - 1;
+    crec[j] = ChanRec{};
+    crec[j].c = c;
+    crec[j].e0 = edges[c];
+    crec[j].e1 = edges[c + 1];
+    crec[j].e2 = last ? edges[c + 1] : edges[c + 2];  // [e1, e2) empty: no channel above the last
+    crec[j].pf0 = prefac[c];
+    crec[j].pf1 = last ? 0.0 : prefac[c + 1];
+  }
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_crec, n_guess * sizeof(ChanRec)));
+  HIP_OR_FAIL(h, hipMemcpy(h->d_crec, crec.data(), n_guess * sizeof(ChanRec), hipMemcpyHostToDevice));
 
   if (h->prec) {
     std::vector<cx<double>> tw;
@@ -383,7 +394,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     m.nu_hi = d->nu_max;
   }
   m.prefac = h->d_prefac;
-  m.guess = h->d_guess;
+  m.crec = h->d_crec;
   m.n_guess = n_guess;
   m.g_lo = g_lo;
   m.g_inv = g_inv;

@@ -87,16 +87,17 @@ struct RTables {
   double z_last, d_last, v_last;
 };
 
+// Branch-free: the node index is clamped into the table for every r (NaN gives node 0), so the three
+// loads are always in bounds and are issued together, and the clamped cases are selected afterwards.
+// For 0 < r < r_last the node is (int)(r idr), at most NZ - 2 (r within one ulp of the last node).
 __device__ __forceinline__ double tab_lerp(const double *y, const double *rr, double idr, double r,
                                            double r_last, double y_last, double y_at0)
 {
 #pragma clang fp contract(off)
-  if (r <= 0) return y_at0;
-  if (r >= r_last) return y_last;
-  int ir = (int)(r * idr);
-  if (ir > 5001 - 2) ir = 5001 - 2;  // r within one ulp of the last node: stay inside the table
-  double y0 = y[ir], y1 = y[ir + 1];
-  return y0 + (y1 - y0) * (r - rr[ir]) * idr;
+  const int ir = (int)fmin(fmax(r * idr, 0.0), (double)(5001 - 2));
+  const double y0 = y[ir], y1 = y[ir + 1], r0 = rr[ir];
+  const double v = y0 + (y1 - y0) * (r - r0) * idr;
+  return r <= 0 ? y_at0 : (r >= r_last ? y_last : v);
 }
 
 __device__ __forceinline__ double z_of_r(const RTables &t, double r)
@@ -138,20 +139,42 @@ __device__ __forceinline__ int channel_of(double nu, int n_nu, int nu_mode, cons
   return lo;
 }
 
-// Same result as channel_of, starting from a guess table over [g_lo, g_lo + n_guess/g_inv):
-// guess[j] is the channel of g_lo + j/g_inv, so the walk below takes at most a step or two.
-__device__ __forceinline__ int channel_of_guess(double nu, int n_nu, int nu_mode, const double *edges,
-                                                double nu_min, double inv_dnu, const int *guess,
-                                                int n_guess, double g_lo, double g_inv)
+// Per-bucket channel record of the TABLE mode.  Bucket j covers the frequencies with
+// (int)((nu - g_lo) g_inv) = j; c is the channel holding g_lo + j / g_inv, e0..e2 are edges[c..c+2]
+// (e2 = e1 for the last channel) and pf0, pf1 the mK factors of channels c and c + 1.  One record
+// (one 64-byte line) decides the channel and its factor of almost every sample with a single
+// dependent load; frequencies outside [e0, e2) walk the edges from c.
+struct __align__(64) ChanRec {
+  double e0, e1, e2, pf0, pf1;
+  int c, pad0;
+  double pad1;
+};
+
+// channel_of for nu_mode 0, and the channel's mK factor; -1 outside [nu_lo, nu_hi) = [edges[0], edges[n_nu])
+__device__ __forceinline__ int channel_of_rec(double nu, int n_nu, const double *edges, const double *prefac,
+                                              const ChanRec *rec, int n_guess, double g_lo, double g_inv,
+                                              double nu_lo, double nu_hi, double &pf)
 {
 #pragma clang fp contract(off)
-  if (nu_mode == 1) return channel_of(nu, n_nu, nu_mode, edges, nu_min, inv_dnu);
-  if (!(nu >= edges[0]) || !(nu < edges[n_nu])) return -1;
-  double t = (nu - g_lo) * g_inv;
-  int j = t < 0 ? 0 : (t >= (double)(n_guess - 1) ? n_guess - 1 : (int)t);
-  int inu = guess[j];
-  while (inu > 0 && nu < edges[inu]) inu--;
-  while (inu < n_nu - 1 && nu >= edges[inu + 1]) inu++;
+  const double t = (nu - g_lo) * g_inv;
+  const int j = (int)fmin(fmax(t, 0.0), (double)(n_guess - 1));
+  const ChanRec r = rec[j];
+  int inu;
+  if (nu >= r.e0 && nu < r.e1) {
+    inu = r.c;
+    pf = r.pf0;
+  }
+  else if (nu >= r.e1 && nu < r.e2) {
+    inu = r.c + 1;
+    pf = r.pf1;
+  }
+  else {
+    inu = r.c;
+    while (inu > 0 && nu < edges[inu]) inu--;
+    while (inu < n_nu - 1 && nu >= edges[inu + 1]) inu++;
+    pf = prefac[inu];
+  }
+  if (!(nu >= nu_lo) || !(nu < nu_hi)) inu = -1;
   return inu;
 }
 

@@ -131,7 +131,7 @@ struct MapParams {
   const double *prefac;    // [n_nu] mK factor per channel (pixelize.c:399-409)
   int64_t nside, npix;
   const double *sig;       // device [sigma2, mean]
-  const int *guess;        // [n_guess] channel of g_lo + j/g_inv (table mode)
+  const ChanRec *crec;     // [n_guess] channel records of the buckets of [g_lo, g_lo + n_guess/g_inv) (table mode)
   int n_guess;
   double g_lo, g_inv;
   const uint32_t *tile_order;  // tiles by increasing distance of their centre from the observer

@@ -79,8 +79,12 @@ __device__ __forceinline__ void cell_hi(const MapParams &m, double delta, double
   s.mass = m.mass_prefac * fraction_hi(redshift) * dens_ln;
 }
 
-// channel and pixel of one sample (pixelize.c:360-375); returns the channel or -1
-__device__ __forceinline__ int sample_bin(const MapParams &m, const CellState &s, int isub, int64_t &ipix)
+// channel and pixel of one sample (pixelize.c:360-375); returns the channel or -1, with the channel's
+// mK factor in pf.  The pixel is computed only for binned samples: computing it for every sample, to
+// overlap its arithmetic with the channel's table loads, needs more VGPRs than five waves per SIMD allow,
+// and four waves are slower (75 against 70 ms at 1024^3).
+__device__ __forceinline__ int sample_bin(const MapParams &m, const CellState &s, int isub, int64_t &ipix,
+                                          double &pf)
 {
   double x = s.x0 + m.sub_off[3 * isub];
   double y = s.y0 + m.sub_off[3 * isub + 1];
@@ -88,8 +92,14 @@ __device__ __forceinline__ int sample_bin(const MapParams &m, const CellState &s
   double r = sqrt(x * x + y * y + z * z);
   double redshift = z_of_r(m.tab, r) + s.dz;
   double nu = 1420.40575177 / (1 + redshift);
-  int inu = channel_of_guess(nu, m.n_nu, m.nu_mode, m.edges, m.nu_min, m.inv_dnu, m.guess, m.n_guess,
-                             m.g_lo, m.g_inv);
+  int inu;
+  if (m.nu_mode == 1) {
+    inu = channel_of(nu, m.n_nu, 1, m.edges, m.nu_min, m.inv_dnu);
+    pf = m.prefac[inu < 0 ? 0 : inu];
+  }
+  else {
+    inu = channel_of_rec(nu, m.n_nu, m.edges, m.prefac, m.crec, m.n_guess, m.g_lo, m.g_inv, m.nu_lo, m.nu_hi, pf);
+  }
   if (!(r > 0)) inu = -1;  // a sample on the observer has no direction
   ipix = inu >= 0 ? vec2pix_ring(m.nside, x, y, z, r) : -1;
   return inu;
@@ -175,10 +185,11 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
     double mass_sub = s.mass / m.n_sub;
     for (int isub = 0; isub < m.n_sub; isub++) {
       int64_t ipix;
-      int inu = sample_bin(m, s, isub, ipix);
+      double pf;
+      int inu = sample_bin(m, s, isub, ipix, pf);
       if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix)
-        agg_add<T, SLOTS>((unsigned long long)ipix + (unsigned long long)m.npix * inu,
-                          (T)(mass_sub * m.prefac[inu]), keys, vals, maps);
+        agg_add<T, SLOTS>((unsigned long long)ipix + (unsigned long long)m.npix * inu, (T)(mass_sub * pf), keys,
+                          vals, maps);
     }
   }
   __syncthreads();
@@ -456,7 +467,8 @@ __global__ __launch_bounds__(256) void k_sample_indices(const cx<T> *__restrict_
     cell_hi(m, (double)F[cell].x, m.sig[0], s);
     for (int isub = 0; isub < m.n_sub; isub++) {
       int64_t ipix;
-      int inu = sample_bin(m, s, isub, ipix);
+      double pf;
+      int inu = sample_bin(m, s, isub, ipix, pf);
       inu_out[c * m.n_sub + isub] = inu;
       ipix_out[c * m.n_sub + isub] = ipix;
     }

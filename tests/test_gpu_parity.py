@@ -99,6 +99,33 @@ def test_sample_indices_bitexact(cfgdir, kw):
     assert np.array_equal(ipix, oipix)
 
 
+def test_sample_indices_irregular_channels(cfgdir):
+    """Channels of very uneven width: the narrowest are far below the channel-lookup buckets (1/8 of
+    the mean width), so several edges share a bucket and the lookup walks the edges."""
+    import os
+    import crime_amd.configs as cf
+    ini = cf.make(cfgdir, "idx_irregular", n_grid=32, n_side=16, n_nu=16, nu_lo=1200.0, nu_hi=1350.0,
+                  l_box=600.0, seed=1001, precision="double", bias_model="reference", n_subcell=0)
+    e = 1350.0 - 150.0 * np.linspace(0.0, 1.0, 17)[::-1] ** 3   # widths 26.4 MHz down to 0.037 MHz
+    np.savetxt(os.path.join(cfgdir, "idx_irregular_nu.txt"), e, fmt="%.10f")
+    par = crime_amd.Params(ini)
+    d = par.desc()
+    width = np.ctypeslib.as_array(d.nuf, shape=(d.n_nu,)) - np.ctypeslib.as_array(d.nu0, shape=(d.n_nu,))
+    assert width.min() < 0.05 < 150.0 / 128   # narrower than one bucket
+    od, ov, os2, _ = oracle.fields(d)
+    orv = oracle.radial_velocity(d, ov)
+    _, odz = oracle.get_hi(d, od, orv, os2)
+    with crime_amd.GetHI(par) as gh:
+        gh.set_grid(crime_amd.GRID_DENS, od)
+        gh.set_grid(crime_amd.GRID_VPOT, ov)
+        gh.set_sigma2(os2)
+        inu, ipix = gh.sample_indices(0, 32 ** 3)
+    oinu, oipix = oracle.sample_indices(d, odz, 0, 32 ** 3)
+    assert np.bincount(oinu[oinu >= 0], minlength=16).min() > 10  # the narrow channels receive samples
+    assert np.array_equal(inu, oinu)
+    assert np.array_equal(ipix, oipix)
+
+
 def test_maps_fp64_vs_oracle(cfgdir):
     par = small_params(cfgdir, "maps64", n_grid=48, n_side=16)
     um = unit_modes(par, 1001)
