@@ -83,12 +83,18 @@ __device__ __forceinline__ void cell_hi(const MapParams &m, double delta, double
 // mK factor in pf.  The pixel is computed only for binned samples: computing it for every sample, to
 // overlap its arithmetic with the channel's table loads, needs more VGPRs than five waves per SIMD allow,
 // and four waves are slower (75 against 70 ms at 1024^3).
+// The sample offsets are the same for every lane and never written by a kernel: read through the
+// constant address space they are scalar loads (with a generic pointer the atomics in the loop keep the
+// compiler from proving that, and every sample issued two vector loads).
+typedef const __attribute__((address_space(4))) double const_double;
+
 __device__ __forceinline__ int sample_bin(const MapParams &m, const CellState &s, int isub, int64_t &ipix,
                                           double &pf)
 {
-  double x = s.x0 + m.sub_off[3 * isub];
-  double y = s.y0 + m.sub_off[3 * isub + 1];
-  double z = s.z0 + m.sub_off[3 * isub + 2];
+  const const_double *so = (const const_double *)m.sub_off;
+  double x = s.x0 + so[3 * isub];
+  double y = s.y0 + so[3 * isub + 1];
+  double z = s.z0 + so[3 * isub + 2];
   double r = sqrt(x * x + y * y + z * z);
   double redshift = z_of_r(m.tab, r) + s.dz;
   double nu = 1420.40575177 / (1 + redshift);
@@ -125,8 +131,9 @@ __device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned 
                                         T *__restrict__ maps)
 {
   const unsigned long long EMPTY = ~0ull;
-  constexpr int SHIFT = 64 - __builtin_ctz(SLOTS);  // Fibonacci hashing onto the table
-  unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> SHIFT) & (SLOTS - 1);
+  // 32-bit Fibonacci hashing of the folded key (one v_mul_lo_u32 instead of a 64-bit product)
+  constexpr int SHIFT = 32 - __builtin_ctz(SLOTS);
+  unsigned h = (((unsigned)key ^ (unsigned)(key >> 32)) * 0x9E3779B1u) >> SHIFT;
   for (int p = 0; p < 64; p++) {
     unsigned long long k = keys[h];
     if (k == EMPTY) {
@@ -188,8 +195,9 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
       double pf;
       int inu = sample_bin(m, s, isub, ipix, pf);
       if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix)
-        agg_add<T, SLOTS>((unsigned long long)ipix + (unsigned long long)m.npix * inu, (T)(mass_sub * pf), keys,
-                          vals, maps);
+        // npix = 12 Nside^2 < 2^32: one 32 x 32 -> 64-bit multiply-add
+        agg_add<T, SLOTS>((unsigned long long)(unsigned)inu * (unsigned)m.npix + (unsigned)ipix, (T)(mass_sub * pf),
+                          keys, vals, maps);
     }
   }
   __syncthreads();
