@@ -141,6 +141,31 @@ def test_maps_fp64_vs_oracle(cfgdir):
     assert rel(m, om) < 1e-10
 
 
+def test_maps_every_aggregation_table_vs_oracle(cfgdir):
+    """The three LDS aggregation tables (2048, 1024, 4096 bins) on a box whose tiles touch more bins than
+    the smaller tables hold: near the observer a cell covers many Nside-256 pixels, so samples overflow
+    to direct HBM atomics.  Every table gives the oracle's maps."""
+    par = small_params(cfgdir, "aggtab", n_grid=32, n_side=256, n_nu=8, nu_lo=1250.0, nu_hi=1400.0)
+    d = par.desc()
+    od, ov, os2, _ = oracle.fields(d)
+    orv = oracle.radial_velocity(d, ov)
+    _, odz = oracle.get_hi(d, od, orv, os2)
+    inu, ipix = oracle.sample_indices(d, odz, 0, 32 ** 3)
+    cell = np.arange(32 ** 3)
+    tile = (cell % 32) // 16 + 2 * ((cell // 32) % 32 // 4 + 8 * (cell // 1024 // 4))
+    key = np.where(inu >= 0, inu.astype(np.int64) * 12 * 256 ** 2 + ipix, -1)
+    bins = [np.unique(key[tile == t][key[tile == t] >= 0]).size for t in range(tile.max() + 1)]
+    assert max(bins) > 2048          # the default and the small table overflow
+    om, _, _ = oracle.run(d)
+    with crime_amd.GetHI(par) as gh:
+        for v in (0, 1, 2):
+            gh.set_schedule([0, 0, 0, v])
+            gh.run()
+            m = gh.maps(np.float64)
+            assert np.array_equal(m != 0, om != 0), v
+            assert rel(m, om) < 1e-10, v
+
+
 def test_maps_fp32_vs_fp64_oracle(cfgdir):
     par = small_params(cfgdir, "maps32", n_grid=64, n_side=32, precision="single")
     with crime_amd.GetHI(par) as gh:
