@@ -746,18 +746,26 @@ __global__ __launch_bounds__(256) void k_alm_unpack(const double *__restrict__ a
 template <int CT>
 __global__ __launch_bounds__(256) void k_syn_gemm(const double *__restrict__ lam, const double *__restrict__ alm,
                                                   double *__restrict__ G, const int4 *__restrict__ tiles, int ntiles,
-                                                  int ncoltiles, int tiles_per_xcd, const int64_t *__restrict__ off,
+                                                  int ncoltiles, const int64_t *__restrict__ off,
                                                   int m0, int lmax, int R, int RP, int ldg, int ncols, int ldo)
 {
   constexpr int RW = ALM_ROWS / 2, NI = RW / 16;
   constexpr int CW = CT / 2, NJ = CW / 16;
   constexpr int AL = ALM_ROWS / 16;                 // rings loaded per thread
   constexpr int BL = CT / 16;                       // columns loaded per thread
+  // XCD-balanced order.  A unit is one (m, parity): its ring tiles x column tiles share the alm rows
+  // (B), and the column tiles of one ring tile share the Legendre rows (A).  Units are dealt to the
+  // eight XCDs in turn (block b runs on XCD b mod 8), so each XCD's L2 serves whole units and every XCD
+  // gets every eighth unit: K (the l rows of the unit) falls with m, and contiguous ranges of units
+  // would leave the XCD with the smallest m doing most of the work.
   const int b = blockIdx.x;
-  const int logical = (b % 8) * tiles_per_xcd + b / 8;
-  if (logical >= ntiles * ncoltiles) return;
-  const int4 tl = tiles[logical / ncoltiles];
-  const int ct = logical % ncoltiles;
+  const int nringtiles = (R + ALM_ROWS - 1) / ALM_ROWS, UB = nringtiles * ncoltiles;
+  const int j = b >> 3, r = j % UB;
+  const int unit = (j / UB) * 8 + (b & 7);
+  const int group = unit * nringtiles + r / ncoltiles;
+  if (group >= ntiles) return;
+  const int4 tl = tiles[group];
+  const int ct = r % ncoltiles;
   const int m = tl.x, par = tl.y, rt = tl.z, nrows = tl.w;
   __shared__ double As[GK][ALM_ROWS + 2];   // [k][ring]: A stored as loaded, read down a column
   __shared__ double Bs[GK][CT + 2];         // [k][column]
@@ -1099,14 +1107,17 @@ hipError_t launch_syn_gemm(const double *lam, const double *alm, double *G, cons
   // SYN_COLS-column tiles unless the columns fit one 64-column tile
   const int ct = ncols > SYN_COLS / 2 ? SYN_COLS : 64;
   const int ncoltiles = (ncols + ct - 1) / ct;
-  const long total = (long)ntiles * ncoltiles;
-  const int per_xcd = (int)((total + 7) / 8);
+  // the tile list holds every ring tile of each (m, parity) unit, consecutively
+  const int nringtiles = (R + ALM_ROWS - 1) / ALM_ROWS;
+  const int nunits = ntiles / nringtiles;
+  if (nunits * nringtiles != ntiles) return hipErrorInvalidValue;
+  const int per_xcd = (nunits + 7) / 8 * nringtiles * ncoltiles;
   if (ct == 64)
     hipLaunchKernelGGL(k_syn_gemm<64>, dim3(8 * per_xcd), dim3(256), 0, s, lam, alm, G, tiles, ntiles, ncoltiles,
-                       per_xcd, off, m0, lmax, R, RP, ldg, ncols, ldo);
+                       off, m0, lmax, R, RP, ldg, ncols, ldo);
   else
     hipLaunchKernelGGL(k_syn_gemm<SYN_COLS>, dim3(8 * per_xcd), dim3(256), 0, s, lam, alm, G, tiles, ntiles,
-                       ncoltiles, per_xcd, off, m0, lmax, R, RP, ldg, ncols, ldo);
+                       ncoltiles, off, m0, lmax, R, RP, ldg, ncols, ldo);
   return hipGetLastError();
 }
 
