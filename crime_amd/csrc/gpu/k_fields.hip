@@ -136,7 +136,12 @@ __global__ __launch_bounds__((FillCfg<T, N, V>::NT)) void k_fill_z(cx<T> *__rest
   using C = FillCfg<T, N, V>;
   __shared__ cx<T> lds[N * C::TX];
   const int c = threadIdx.x % C::TX, t = threadIdx.x / C::TX;
-  const int kx = blockIdx.x * C::TX + c, ky = blockIdx.y;
+  // Blocks go to the XCDs round-robin (XCD = linear block index mod 8), and N / TX is a multiple of 8,
+  // so without a rotation every row's kx = 0 and kx = N/2 blocks -- whose waves run the one-draw and the
+  // two-draw (Hermitian) branches one after the other, three draws in all -- would land on XCD 0.
+  // Rotating the column block by the row spreads them over all eight.
+  const int xb = (blockIdx.x + blockIdx.y) % gridDim.x;
+  const int kx = xb * C::TX + c, ky = blockIdx.y;
   const size_t n2 = (size_t)N * N;
   cx<T> *col = F + (size_t)kx + (size_t)N * ky;
   // the generator is too large to unroll into the register tile: stage the modes in LDS first
