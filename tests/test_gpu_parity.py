@@ -99,15 +99,17 @@ def test_sample_indices_bitexact(cfgdir, kw):
     assert np.array_equal(ipix, oipix)
 
 
-def test_sample_indices_irregular_channels(cfgdir):
+@pytest.mark.parametrize("seed", [1001, 7, 424242])
+def test_sample_indices_irregular_channels(cfgdir, seed):
     """Channels of very uneven width: the narrowest are far below the channel-lookup buckets (1/8 of
     the mean width), so several edges share a bucket and the lookup walks the edges."""
     import os
     import crime_amd.configs as cf
-    ini = cf.make(cfgdir, "idx_irregular", n_grid=32, n_side=16, n_nu=16, nu_lo=1200.0, nu_hi=1350.0,
-                  l_box=600.0, seed=1001, precision="double", bias_model="reference", n_subcell=0)
+    name = f"idx_irregular_{seed}"
+    ini = cf.make(cfgdir, name, n_grid=32, n_side=16, n_nu=16, nu_lo=1200.0, nu_hi=1350.0,
+                  l_box=600.0, seed=seed, precision="double", bias_model="reference", n_subcell=0)
     e = 1350.0 - 150.0 * np.linspace(0.0, 1.0, 17)[::-1] ** 3   # widths 26.4 MHz down to 0.037 MHz
-    np.savetxt(os.path.join(cfgdir, "idx_irregular_nu.txt"), e, fmt="%.10f")
+    np.savetxt(os.path.join(cfgdir, f"{name}_nu.txt"), e, fmt="%.10f")
     par = crime_amd.Params(ini)
     d = par.desc()
     width = np.ctypeslib.as_array(d.nuf, shape=(d.n_nu,)) - np.ctypeslib.as_array(d.nu0, shape=(d.n_nu,))
