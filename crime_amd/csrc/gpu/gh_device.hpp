@@ -150,15 +150,19 @@ struct __align__(64) ChanRec {
   double pad1;
 };
 
-// channel_of for nu_mode 0, and the channel's mK factor; -1 outside [nu_lo, nu_hi) = [edges[0], edges[n_nu])
-__device__ __forceinline__ int channel_of_rec(double nu, int n_nu, const double *edges, const double *prefac,
-                                              const ChanRec *rec, int n_guess, double g_lo, double g_inv,
-                                              double nu_lo, double nu_hi, double &pf)
+// bucket of a frequency: (int)((nu - g_lo) g_inv), clamped into the table (NaN gives bucket 0)
+__device__ __forceinline__ int channel_bucket(double nu, int n_guess, double g_lo, double g_inv)
 {
 #pragma clang fp contract(off)
   const double t = (nu - g_lo) * g_inv;
-  const int j = (int)fmin(fmax(t, 0.0), (double)(n_guess - 1));
-  const ChanRec r = rec[j];
+  return (int)fmin(fmax(t, 0.0), (double)(n_guess - 1));
+}
+
+// channel_of for nu_mode 0 given the record r of nu's bucket, and the channel's mK factor; -1 outside
+// [nu_lo, nu_hi) = [edges[0], edges[n_nu])
+__device__ __forceinline__ int channel_of_rec(double nu, const ChanRec &r, int n_nu, const double *edges,
+                                              const double *prefac, double nu_lo, double nu_hi, double &pf)
+{
   int inu;
   if (nu >= r.e0 && nu < r.e1) {
     inu = r.c;
