@@ -388,7 +388,7 @@ def main():
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "traffic": measured_traffic(args.workload).get(KERNEL_OF_STAGE[dom]),
                          "bytes_per_launch": by[dom], "ms_per_launch": launch[dom],
-                         "note": ("pixelize is bound by VALU issue, not HBM: 81% of each SIMD's cycles issue VALU "
+                         "note": ("pixelize is bound by VALU issue, not HBM: 84% of each SIMD's cycles issue VALU "
                                   "at 1024^3 (profiles/sq_counters_pixelize_production_1024.md)")
                          if dom == "pixelize" else None},
             "cpu_baseline": cpu,
