@@ -79,16 +79,16 @@ __device__ __forceinline__ void cell_hi(const MapParams &m, double delta, double
   s.mass = m.mass_prefac * fraction_hi(redshift) * dens_ln;
 }
 
-// Channel and pixel of one sample (pixelize.c:360-375), in two halves so that the per-sample loop can
-// interleave two samples: sample_pre computes the position, r, the frequency and loads the channel
-// record (table mode); sample_post decides the channel and the pixel and returns the channel or -1,
-// with the channel's mK factor in pf.  The pixel is computed only for binned samples: computing it for
-// every sample needs more VGPRs than the occupancy allows.
 // The sample offsets are the same for every lane and never written by a kernel: read through the
 // constant address space they are scalar loads (with a generic pointer the atomics in the loop keep the
 // compiler from proving that, and every sample issued two vector loads).
 typedef const __attribute__((address_space(4))) double const_double;
 
+// Channel and pixel of one sample (pixelize.c:360-375), in two halves so that the per-sample loop can
+// interleave two samples: sample_pre computes the position, r, the frequency and loads the channel
+// record (table mode); sample_post decides the channel and the pixel and returns the channel or -1,
+// with the channel's mK factor in pf.  The pixel is computed only for binned samples: computing it for
+// every sample needs more VGPRs than the occupancy allows.
 struct SamplePre {
   double x, y, z, r, nu;
   ChanRec rec;  // table mode only
