@@ -13,6 +13,9 @@
 #include "gh_fft.hpp"
 #include "gh_sht.hpp"
 
+#include <algorithm>
+#include <vector>
+
 namespace crime {
 
 // Layout of the ring spectra G: blocks of 16 rings, then m, then parity (E, O), then the 16 rings of
@@ -618,12 +621,20 @@ __global__ __launch_bounds__(256) void k_alm_gemm(const double *__restrict__ lam
 // X[a][k] = w_m alm_a, Y[b][k] = alm_b, w_0 = 1, w_m = 2, for the pairs within each of nsets sets of
 // mps maps (one set: every pair).  One workgroup per (set, 64 x 64 map tile with ta <= tb, l);
 // off-diagonal tiles also write the mirror block, so C^{ab} and C^{ba} are equal bit for bit.
-// Consecutive blocks of one XCD take consecutive l of one tile, so the [set][a][b][l] stores of
-// neighbouring l meet in that XCD's L2.
+// Each XCD takes one contiguous run of logical blocks [xr.start[x], xr.start[x + 1]), so consecutive
+// blocks of one XCD take consecutive l of one tile and the [set][a][b][l] stores of neighbouring l meet
+// in that XCD's L2.  A block's work grows with l, so the host cuts the runs at equal work, not at equal
+// block counts (launch_cls_sets).
+struct XcdRuns {
+  int start[9];
+};
+
 __global__ __launch_bounds__(256) void k_cls_cross(const double *__restrict__ alm, int lmax, int mps, int ldo,
-                                                   int ntile, double *__restrict__ cls)
+                                                   int ntile, XcdRuns xr, double *__restrict__ cls)
 {
-  const int logical = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int x = blockIdx.x & 7;   // blocks are dealt to the 8 XCDs round robin
+  const int logical = xr.start[x] + (int)(blockIdx.x >> 3);
+  if (logical >= xr.start[x + 1]) return;
   const int nl = lmax + 1;
   const int l = lmax - logical % nl;
   const int npair = ntile * (ntile + 1) / 2;
@@ -1076,9 +1087,33 @@ hipError_t launch_alm_gemm(const double *lam, const double *G, const int4 *tiles
 hipError_t launch_cls_sets(const double *alm, int lmax, int nsets, int mps, int ldo, double *cls, hipStream_t s)
 {
   const int nt = (mps + GT - 1) / GT;
-  const int64_t nblk = (int64_t)nsets * nt * (nt + 1) / 2 * (lmax + 1);
+  const int nl = lmax + 1;
+  const int64_t ngroups = (int64_t)nsets * nt * (nt + 1) / 2;   // (set, tile pair) groups of nl blocks
+  const int64_t nblk = ngroups * nl;
   if (nblk > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_cls_cross, dim3((unsigned)nblk), dim3(256), 0, s, alm, lmax, mps, ldo, nt, cls);
+  // cost of logical block j of a group (l = lmax - j): its m chunks of 8 plus two for loads and stores
+  std::vector<int64_t> pre(nl + 1, 0);
+  for (int j = 0; j < nl; j++) pre[j + 1] = pre[j] + (lmax - j + 8) / 8 + 2;
+  const int64_t wg = pre[nl];
+  const double total = (double)wg * (double)ngroups;
+  XcdRuns xr;
+  xr.start[0] = 0;
+  xr.start[8] = (int)nblk;
+  for (int x = 1; x < 8; x++) {
+    const double target = total * x / 8.0;
+    int64_t q = (int64_t)(target / (double)wg);
+    if (q > ngroups) q = ngroups;
+    const int64_t rem = (int64_t)(target - (double)q * (double)wg);
+    const int64_t i = std::lower_bound(pre.begin(), pre.end(), rem) - pre.begin();
+    int64_t b = q * nl + (i < nl ? i : nl);
+    if (b > nblk) b = nblk;
+    if (b < xr.start[x - 1]) b = xr.start[x - 1];
+    xr.start[x] = (int)b;
+  }
+  int per_xcd = 0;
+  for (int x = 0; x < 8; x++) per_xcd = std::max(per_xcd, xr.start[x + 1] - xr.start[x]);
+  if (per_xcd == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cls_cross, dim3((unsigned)(8 * per_xcd)), dim3(256), 0, s, alm, lmax, mps, ldo, nt, xr, cls);
   return hipGetLastError();
 }
 
