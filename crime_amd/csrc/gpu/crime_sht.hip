@@ -39,6 +39,10 @@ struct crime_sht {
   int64_t nalm = 0;
   size_t lds = 0;
   hipStream_t stream = nullptr;
+  // the ring groups write disjoint rings of G: all but every third run on two side streams, forked from
+  // and joined back into `stream`, so that the many short Bluestein and split-DFT launches overlap
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
   ShtRing *d_rings = nullptr;
   double *d_z = nullptr, *d_st = nullptr;
   double2 *d_coef = nullptr;
@@ -104,6 +108,12 @@ extern "C" void crime_sht_destroy(crime_sht_t *p)
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; k++) {
+    if (p->side[k]) (void)hipStreamSynchronize(p->side[k]);
+    if (p->join[k]) (void)hipEventDestroy(p->join[k]);
+    if (p->side[k]) (void)hipStreamDestroy(p->side[k]);
+  }
+  if (p->fork) (void)hipEventDestroy(p->fork);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
@@ -277,6 +287,11 @@ static int create_impl(crime_sht *p, int64_t nside, int lmax, int max_maps, int 
   if (device < 0 || device >= ndev) return sht_error(p, CRIME_GH_ENODEV, "device %d of %d", device, ndev);
   SHT_HIP(p, hipSetDevice(device));
   SHT_HIP(p, hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+  for (int k = 0; k < 2; k++) {
+    SHT_HIP(p, hipStreamCreateWithFlags(&p->side[k], hipStreamNonBlocking));
+    SHT_HIP(p, hipEventCreateWithFlags(&p->join[k], hipEventDisableTiming));
+  }
+  SHT_HIP(p, hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
   SHT_HIP(p, sht_set_lds_limits(p->lds));
 
   std::vector<ShtRing> rings;
@@ -406,6 +421,26 @@ static ShtLaunch ring_launch(const crime_sht *p, ShtLaunch g, int nmaps)
   return g;
 }
 
+// Launch every ring group through `launch` (called with the group and the stream to use): group i on
+// the plan's stream when i % 3 == 0, else on side stream i % 3 - 1.  The side streams start after the
+// work already queued on the plan's stream, and the plan's stream continues after both.
+template <class Launch>
+static int run_ring_groups(crime_sht *p, Launch launch)
+{
+  SHT_HIP(p, hipEventRecord(p->fork, p->stream));
+  for (int k = 0; k < 2; k++) SHT_HIP(p, hipStreamWaitEvent(p->side[k], p->fork, 0));
+  for (size_t i = 0; i < p->groups.size(); i++) {
+    const hipStream_t s = i % 3 == 0 ? p->stream : p->side[i % 3 - 1];
+    const int rc = launch(p->groups[i], s);
+    if (rc) return rc;
+  }
+  for (int k = 0; k < 2; k++) {
+    SHT_HIP(p, hipEventRecord(p->join[k], p->side[k]));
+    SHT_HIP(p, hipStreamWaitEvent(p->stream, p->join[k], 0));
+  }
+  return 0;
+}
+
 // ring spectra + alm GEMMs of nsets sets of mps device maps each (set s at d_maps[s]) into p->d_alm,
 // columns 2 (s mps + t) + (re | im); stage times into p->st
 static int analyse_sets(crime_sht *p, int nsets, const void *const *d_maps, int mps, int fp64)
@@ -421,11 +456,14 @@ static int analyse_sets(crime_sht *p, int nsets, const void *const *d_maps, int 
   hipEvent_t *ev = p->ev.data();
   SHT_HIP(p, hipMemcpyAsync(p->d_mtab, p->h_mtab.data(), nsets * sizeof(void *), hipMemcpyHostToDevice, p->stream));
   SHT_HIP(p, hipEventRecord(ev[0], p->stream));
-  for (const ShtLaunch &g : p->groups) {
+  int rc = run_ring_groups(p, [&](const ShtLaunch &g, hipStream_t s) -> int {
     ShtLaunch a = ring_launch(p, g, nmaps);
     a.mps = mps;
+    a.s = s;
     SHT_HIP(p, launch_rings(a, (const void *const *)p->d_mtab, fp64));
-  }
+    return 0;
+  });
+  if (rc) return rc;
   SHT_HIP(p, hipEventRecord(ev[1], p->stream));
   for (size_t i = 0; i < p->chunks.size(); i++) {
     const Chunk &c = p->chunks[i];
@@ -619,7 +657,13 @@ static int synthesize(crime_sht *p, int nmaps, void *d_out, int fp64)
                                p->R, p->RP, p->ldg, 2 * nmaps, p->ldo, p->stream));
     SHT_HIP(p, hipEventRecord(ev[2 + 2 * i], p->stream));
   }
-  for (const ShtLaunch &g : p->groups) SHT_HIP(p, launch_syn_rings(ring_launch(p, g, nmaps), d_out, fp64));
+  int rc = run_ring_groups(p, [&](const ShtLaunch &g, hipStream_t s) -> int {
+    ShtLaunch a = ring_launch(p, g, nmaps);
+    a.s = s;
+    SHT_HIP(p, launch_syn_rings(a, d_out, fp64));
+    return 0;
+  });
+  if (rc) return rc;
   SHT_HIP(p, hipEventRecord(ev[1 + 2 * p->chunks.size()], p->stream));
   return 0;
 }
