@@ -13,7 +13,9 @@
  *   orc_mk_T_maps         src/pixelize.c:303-414 with get_inu :181-208 (the warm-started walk)
  *   orc_vec2pix_ring      chealpix vec2pix_ring (called at src/pixelize.c:375), restated from
  *                         Gorski et al. 2005, ApJ 622:759
- * Parity pins (see tests/): FFT vs brute-force DFT and numpy.fft.irfftn; sigma^2 vs numpy; v_r vs the
+ * Parity pins (see tests/): FFT vs brute-force DFT and numpy.fft.irfftn; the mode fill through single
+ * injected modes against the closed forms delta = norm c A Re(g e^{ik.x}), phi = delta f0 H0 / k^2 and
+ * the smoothing factor e^{-r_s^2 k^2 / 2} (tests/test_oracle_modes.py); sigma^2 vs numpy; v_r vs the
  * closed-form derivative of single Fourier modes; vec2pix vs pix2ang round trips, poles and equal-area
  * counts; channels vs a brute-force scan; maps vs total-mass conservation.  The last bits of GSL's
  * integrators, FFTW's plans and chealpix are not reproducible here (none is vendored): parity with
