@@ -261,7 +261,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   HIP_OR_FAIL(h, hipMemcpy(h->d_pk + d->numk, d->pk, d->numk * sizeof(double), hipMemcpyHostToDevice));
   HIP_OR_FAIL(h, hipMemcpy(h->d_sub, d->sub_off, 3 * (size_t)d->n_sub * sizeof(double), hipMemcpyHostToDevice));
 
-  // channel edges and mK factors (pixelize.c:308, 399-409)
+  // channel edges and mK factors (pixelize.c:155, 246-256)
   std::vector<double> edges(d->n_nu + 1), prefac(d->n_nu);
   double dnu_u = (d->nu_max - d->nu_min) / d->n_nu;
   for (int i = 0; i < d->n_nu; i++) edges[i] = d->nu_mode == CRIME_GH_NU_TABLE ? d->nu0[i] : d->nu_min + dnu_u * i;
@@ -389,7 +389,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     m.nu_hi = edges[d->n_nu];
   }
   else {
-    // (int) truncation bins frequencies down to nu_min - dnu into channel 0 (pixelize.c:369)
+    // (int) truncation bins frequencies down to nu_min - dnu into channel 0 (pixelize.c:216)
     m.nu_lo = d->nu_min - dnu_u;
     m.nu_hi = d->nu_max;
   }
@@ -786,7 +786,7 @@ extern "C" int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes)
 // ---------------------------------------------------------------------------------------------
 // point sources
 // ---------------------------------------------------------------------------------------------
-// setup_psources (src/psources.c:112-145): n(z_i) = sum over 256 log10 L midpoints of dlogL
+// setup_psources (src/psources.c:99-128): n(z_i) = sum over 256 log10 L midpoints of dlogL
 // l_z_function, and max_Lpdf = 1.1 max l_z_function / n(z_i), at z_i = i z_max / 256
 static double ps_lfunc_host(double L, double z)
 {

@@ -725,7 +725,7 @@ extern "C" int crime_sht_alm2map(crime_sht_t *p, int nmaps, const double *alms, 
   return 0;
 }
 
-// FWHM2SIGMA of he_generate_beam_window (src/healpix_extra.c:389): arcmin -> radians / sqrt(8 ln 2)
+// FWHM2SIGMA (src/healpix_extra.c:389), used by he_generate_beam_window (:390-400): arcmin -> radians / sqrt(8 ln 2)
 static const double SHT_FWHM2SIGMA = 0.00012352884853326381;
 
 extern "C" int crime_sht_smooth_device(crime_sht_t *p, int nmaps, void *d_maps, int maps_fp64,

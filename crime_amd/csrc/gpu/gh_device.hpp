@@ -2,7 +2,7 @@
 //
 // Everything that decides an integer (table index, channel, HEALPix pixel) is FP64 with FMA
 // contraction disabled, so that it performs the same IEEE operations in the same order as the
-// host-side restatement of the reference (src/cosmo.c:52-86, src/pixelize.c:181-208, 343-381).
+// host-side restatement of the reference (src/cosmo.c:52-86, src/pixelize.c:28-55, 190-228).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -115,7 +115,7 @@ __device__ __forceinline__ double vgrowth_of_r(const RTables &t, double r)
 
 // ---------------------------------------------------------------------------------------------
 // channel of a frequency.  TABLE mode: the unique i with nu0[i] <= nu < nuf[i] (edges[i] = nu0[i],
-// edges[n] = nuf[n-1]); get_inu's walk (src/pixelize.c:181-208) converges to this index from any
+// edges[n] = nuf[n-1]); get_inu's walk (src/pixelize.c:28-55) converges to this index from any
 // start because the bins are contiguous.  UNIFORM mode: (int)(inv_dnu*(nu-nu_min)) (:369).
 // Returns -1 when no channel receives the sample.
 // ---------------------------------------------------------------------------------------------

@@ -128,7 +128,7 @@ struct MapParams {
   const double *edges;     // [n_nu+1]
   double nu_min, inv_dnu;
   double nu_lo, nu_hi;     // a sample can be binned only if nu_lo <= nu < nu_hi
-  const double *prefac;    // [n_nu] mK factor per channel (pixelize.c:399-409)
+  const double *prefac;    // [n_nu] mK factor per channel (pixelize.c:246-256)
   int64_t nside, npix;
   const double *sig;       // device [sigma2, mean]
   const ChanRec *crec;     // [n_guess] channel records of the buckets of [g_lo, g_lo + n_guess/g_inv) (table mode)
@@ -144,7 +144,7 @@ constexpr int TILE_X = 16, TILE_Y = 4, TILE_Z = 4;
 constexpr int AGG_SLOTS = 2048;
 
 // Point sources (src/grid_tools.c:24-101, src/pixelize.c:58-148, src/psources.c): constants of the
-// reference's toy luminosity function and SED (psources.c:23-63, 147-152) and the per-run tables.
+// reference's toy luminosity function and SED (psources.c:24-64, 154-155) and the per-run tables.
 constexpr int PS_NZ = 256;                      // NZ_PSOURCES (src/common_gh.h:35)
 constexpr double PS_LLOGMIN = -5.0, PS_LLOGMAX = 6.0;
 constexpr int PS_NL = 256;                      // NL_PSOURCES
@@ -157,7 +157,7 @@ constexpr int PS_SLOTS = 1024;                  // pixels of a tile's LDS table 
 struct PsParams {
   const double *nz;          // [PS_NZ] n(z_i) (nz_psources_arr)
   const double *maxpdf;      // [PS_NZ] (max_Lpdf_arr)
-  double dz, inv_dz, z_max;  // glob_dz, glob_inv_dz, z_max (psources.c:117-118, cosmo.c:357)
+  double dz, inv_dz, z_max;  // glob_dz, glob_inv_dz, z_max (psources.c:106-107, cosmo.c:357)
   double cell_vol;
   uint32_t seed;
   double *A, *B;             // [npix] sums of the two SED terms per pixel

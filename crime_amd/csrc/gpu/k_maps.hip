@@ -1,10 +1,10 @@
 // Map stage of the GetHI density-to-sky path on gfx950: get_HI (src/grid_tools.c:103-153) fused
-// with mk_T_maps (src/pixelize.c:303-414) and the radial-velocity stencil (src/fourier.c:307-373).
+// with mk_T_maps (src/pixelize.c:150-261) and the radial-velocity stencil (src/fourier.c:307-373).
 //
 // One thread per cell reads (delta, phi) of the cell and phi of its six periodic neighbours, forms
 // v_r, the lognormal HI mass and the RSD redshift shift in FP64, and scatters its n_sub samples into
 // the (channel, RING pixel) maps with hardware FP32/FP64 atomics, already multiplied by the channel's
-// mass-to-mK factor (the reference scales the finished maps, pixelize.c:399-414; the product of the
+// mass-to-mK factor (the reference scales the finished maps, pixelize.c:246-261; the product of the
 // factor with each sample differs from it only in rounding).  Cells whose whole sample cloud lies
 // outside every channel -- decided with a conservative bound on r and the cell's exact RSD shift --
 // skip the per-sample work; a sample that the reference would bin is never skipped.
@@ -84,7 +84,7 @@ __device__ __forceinline__ void cell_hi(const MapParams &m, double delta, double
 // compiler from proving that, and every sample issued two vector loads).
 typedef const __attribute__((address_space(4))) double const_double;
 
-// Channel and pixel of one sample (pixelize.c:360-375), in two halves so that the per-sample loop can
+// Channel and pixel of one sample (pixelize.c:207-222), in two halves so that the per-sample loop can
 // interleave two samples: sample_pre computes the position, r, the frequency and loads the channel
 // record (table mode); sample_post decides the channel and the pixel and returns the channel or -1,
 // with the channel's mK factor in pf.  The pixel is computed only for binned samples: computing it for
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
 // ---------------------------------------------------------------------------------------------
 // Point sources.  get_point_sources (grid_tools.c:24-101) draws a Poisson number of sources per cell
 // from the Gaussian delta; mk_psources_maps (pixelize.c:58-148) places each uniformly in its cell,
-// draws its luminosity from the toy luminosity function by rejection (psources.c:147-170) and adds
+// draws its luminosity from the toy luminosity function by rejection (psources.c:130-152) and adds
 //   T(nu) = FLUX2TEMP LUM2FLUX 4 pi L SED((1+z) nu) h^2 / (r^2 (1+z) dOmega nu^2)      (:173-184)
 // to every channel of its pixel.  SED = SED_NORM (x^-0.1 + 10 x^-0.75), x = (1+z) nu / 1000, factors
 // into a source part and a channel part, so each source adds two numbers to its pixel,
@@ -320,7 +320,7 @@ __device__ uint32_t ps_poisson(double lam, uint32_t seed, uint64_t cell)
   return (uint32_t)lam;
 }
 
-// n_of_z_psources (psources.c:78-95)
+// n_of_z_psources (psources.c:74-90)
 __device__ __forceinline__ double ps_nofz(const PsParams &p, double z)
 {
   const int iz = (int)(z * p.inv_dz);
@@ -330,7 +330,7 @@ __device__ __forceinline__ double ps_nofz(const PsParams &p, double z)
   return p.nz[iz] + (p.nz[iz + 1] - p.nz[iz]) * (z - zi) * p.inv_dz;
 }
 
-// draw_luminosity (psources.c:147-170): u, then log10 L uniform in [LLOGMIN, LLOGMAX], accepted when
+// draw_luminosity (psources.c:130-152): u, then log10 L uniform in [LLOGMIN, LLOGMAX], accepted when
 // u max_pdf < l_z_function(L, z) / n(z); z >= z_max gives L = 0.  10^x = exp(x ln 10) and
 // (L/LX)^(alpha+1) = exp((alpha+1) ln(L/LX)) are the same functions written with one exponential each.
 // ps_lum_setup: the per-source constants (false when z >= z_max); ps_lum_try: attempt `att`.
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) void k_psources(const cx<T> *__restrict__ F, M
     const double ndens = ps_nofz(ps, zred);
     uint32_t np = 0;
     if (ndens > 0) {
-      const double gfb = dgrowth_of_r(m.tab, rc);   // bias_psources = 1 (psources.c:68-71)
+      const double gfb = dgrowth_of_r(m.tab, rc);   // bias_psources = 1 (psources.c:66-69)
       const double lam = ndens * ps.cell_vol * exp(gfb * ((double)F[cell].x - 0.5 * gfb * m.sig[0]));
       np = ps_poisson(lam, ps.seed, cell);
     }

@@ -76,7 +76,7 @@ void gh_unit_modes_mt(int n, uint32_t seed, double *out)
 int gh_subcell_offsets(int n_subcell, double lcell, uint32_t seed, double *out)
 {
   if (n_subcell <= 0) {
-    /* src/pixelize.c:310-317 */
+    /* src/pixelize.c:157-164 */
     gh_mt s;
     gh_mt_seed(&s, seed);
     for (int ii = 0; ii < GH_N_SUBPART; ii++) {

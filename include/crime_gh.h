@@ -5,11 +5,11 @@
  * ParamGetHI record in place (src/main_gh.c:52-62):
  *     create_d_and_vr_fields(par);   src/fourier.c:375-438
  *     get_HI(par);                   src/grid_tools.c:103-153
- *     mk_T_maps(par);                src/pixelize.c:303-439 (scatter :343-384, mK scaling :399-414)
+ *     mk_T_maps(par);                src/pixelize.c:150-286 (scatter :190-231, mK scaling :246-261)
  * This header is the device-side replacement of that seam.  Everything the three calls read from
- * ParamGetHI (src/common_gh.h:138-209) is handed over once, by value, in crime_gh_desc; the library
+ * ParamGetHI (src/common_gh.h:58-129) is handed over once, by value, in crime_gh_desc; the library
  * keeps the fields and maps resident in HBM and returns the maps in the layout of maps_HI after
- * pixelize.c:412 (maps[ipix + N_pix*inu], RING ordering, channel 0 = lowest frequency, mK).
+ * pixelize.c:259 (maps[ipix + N_pix*inu], RING ordering, channel 0 = lowest frequency, mK).
  *
  * Conventions (differences from the reference are deliberate and listed):
  *  - every entry point returns a status code (0 = OK, <0 = error); nothing calls exit().  The
@@ -31,8 +31,8 @@ extern "C" {
 #endif
 
 #define CRIME_GH_ABI_VERSION 1u
-#define CRIME_GH_NZ 5001            /* table length, NZ in src/common_gh.h:114 */
-#define CRIME_GH_DZ 0.001           /* z step of the z->r table, DZ in src/common_gh.h:113 */
+#define CRIME_GH_NZ 5001            /* table length, NZ in src/common_gh.h:34 */
+#define CRIME_GH_DZ 0.001           /* z step of the z->r table, DZ in src/common_gh.h:33 */
 #define CRIME_GH_NU_21 1420.40575177 /* MHz, NU_21 in src/common.h:38 */
 
 /* status codes */
@@ -58,7 +58,7 @@ enum {
 };
 
 /* frequency binning: table of edges (reference build flag _IRREGULAR_NUTABLE, get_inu
- * src/pixelize.c:181-208) or uniform bins with the reference's (int) truncation (:369) */
+ * src/pixelize.c:28-55) or uniform bins with the reference's (int) truncation (:216) */
 enum { CRIME_GH_NU_TABLE = 0, CRIME_GH_NU_UNIFORM = 1 };
 
 /* HI bias model.  REFERENCE = bias_HI(z)=0.904+0.135(1+z)^1.696 (src/user_defined.c:32-35);
@@ -78,7 +78,7 @@ typedef struct crime_gh_desc {
   double nu_min, nu_max;      /* MHz */
   double l_box;               /* box side, Mpc/h */
   double pos_obs[3];          /* observer position, Mpc/h */
-  double omega_b, hhub;       /* for the mass-to-temperature factor (src/pixelize.c:308) */
+  double omega_b, hhub;       /* for the mass-to-temperature factor (src/pixelize.c:155) */
   double f0h0;                /* fgrowth_0 * hubble_0 (src/fourier.c:255) */
   double r2_smooth;           /* squared smoothing scale (already squared, src/io_gh.c:255-260) */
   int32_t do_smoothing;
@@ -97,7 +97,7 @@ typedef struct crime_gh_desc {
   int32_t bias_model;         /* CRIME_GH_BIAS_* */
   double bias_b0, bias_b1;
   uint32_t seed;              /* Philox key for CRIME_GH_MODES_PHILOX */
-  int32_t n_sub;              /* samples per cell (reference N_SUBPART=10, src/pixelize.c:178) */
+  int32_t n_sub;              /* samples per cell (reference N_SUBPART=10, src/pixelize.c:25) */
   const double *sub_off;      /* [n_sub*3] sample offsets from the cell centre, Mpc/h */
   int32_t mode_source;        /* CRIME_GH_MODES_* */
   const double *unit_modes;   /* [2*N*N*(N/2+1)] interleaved (re,im), index kx+(N/2+1)(ky+N kz);
@@ -156,7 +156,7 @@ const char *crime_gh_last_error(const crime_gh_t *h);
  * velocity stencil (:307-373) is evaluated inside crime_gh_mk_T_maps. */
 int crime_gh_create_d_and_vr_fields(crime_gh_t *h, crime_gh_stats *st);
 
-/* get_HI (src/grid_tools.c:103-153) fused with mk_T_maps (src/pixelize.c:303-414): per cell the
+/* get_HI (src/grid_tools.c:103-153) fused with mk_T_maps (src/pixelize.c:150-261): per cell the
  * radial velocity, lognormal HI mass and RSD shift; per sample the channel and RING pixel; the
  * scatter-add into zeroed maps, scaled to mK per channel. */
 int crime_gh_mk_T_maps(crime_gh_t *h, crime_gh_stats *st);
@@ -208,7 +208,7 @@ int crime_gh_autotune(crime_gh_t *h, int reps, int *sched_out, double *ms_out);
  * for callers that hand the maps to further device work without a host round trip. */
 int crime_gh_device_maps(crime_gh_t *h, void **dptr, size_t *bytes);
 
-/* Point sources (do_psources, src/main_gh.c:55-66): setup_psources (src/psources.c:112-145),
+/* Point sources (do_psources, src/main_gh.c:55-66): setup_psources (src/psources.c:99-128),
  * get_point_sources (src/grid_tools.c:24-101) and mk_psources_maps (src/pixelize.c:58-148) on the
  * resident fields, into point-source maps of their own (maps_PS, written as <prefix>_ps_###.fits by
  * src/io_gh.c:122-127), layout maps_PS[ipix + N_pix * inu], mK, the grid's precision.

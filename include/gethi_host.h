@@ -24,7 +24,7 @@ extern "C" {
 #define GH_DZ CRIME_GH_DZ
 #define GH_NU_21 CRIME_GH_NU_21
 #define GH_HMPC 2997.92458 /* c/H0 in Mpc/h (CSM_HMPC, src/cosmo_mad.h) */
-#define GH_N_SUBPART 10    /* src/pixelize.c:178 */
+#define GH_N_SUBPART 10    /* src/pixelize.c:25 */
 
 typedef struct gh_params {
   /* ---- keys of the reference parameter file (src/io_gh.c:211-251) ---- */
@@ -142,7 +142,7 @@ double gh_mt_uniform(gh_mt *s);
  * zero for k=0.  out has 2*N*N*(N/2+1) doubles. */
 void gh_unit_modes_mt(int n_grid, uint32_t seed, double *out);
 /* sample offsets: n_subcell==0 -> 10 offsets drawn from MT19937(seed) in x,y,z order
- * (src/pixelize.c:310-317); n_subcell=k>0 -> k^3 regular sub-cell centres.  Returns n_sub. */
+ * (src/pixelize.c:157-164); n_subcell=k>0 -> k^3 regular sub-cell centres.  Returns n_sub. */
 int gh_subcell_offsets(int n_subcell, double lcell, uint32_t seed, double *out /* [3*n_sub] */);
 
 /* ---- outputs (src/io_gh.c:69-131, src/healpix_extra.c:132-164) ---- */

@@ -10,8 +10,8 @@
  *   orc_fields            src/fourier.c:375-438 (normalisation :395-412, sigma^2 :24-76)
  *   orc_radial_velocity   src/fourier.c:307-373
  *   orc_get_hi            src/grid_tools.c:103-153, src/user_defined.c:27-35
- *   orc_mk_T_maps         src/pixelize.c:303-414 with get_inu :181-208 (the warm-started walk)
- *   orc_vec2pix_ring      chealpix vec2pix_ring (called at src/pixelize.c:375), restated from
+ *   orc_mk_T_maps         src/pixelize.c:150-261 with get_inu :28-55 (the warm-started walk)
+ *   orc_vec2pix_ring      chealpix vec2pix_ring (called at src/pixelize.c:222), restated from
  *                         Gorski et al. 2005, ApJ 622:759
  * Parity pins (see tests/): FFT vs brute-force DFT and numpy.fft.irfftn; the mode fill through single
  * injected modes against the closed forms delta = norm c A Re(g e^{ik.x}), phi = delta f0 H0 / k^2 and
@@ -379,7 +379,7 @@ void orc_get_hi(const crime_gh_desc *d, const double *dens, const double *rvel, 
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* mk_T_maps: src/pixelize.c:181-208, 303-414                                                  */
+/* mk_T_maps: src/pixelize.c:28-55, 150-261                                                  */
 /* ------------------------------------------------------------------------------------------ */
 static int64_t imod64(int64_t v, int64_t m)
 {
@@ -424,7 +424,7 @@ int64_t orc_vec2pix_ring(int64_t nside, double x, double y, double z)
 
 int orc_get_inu(const crime_gh_desc *d, double nu, int inu_start)
 {
-  /* get_inu, src/pixelize.c:181-208 (table mode); the uniform (int) cast of :369 otherwise */
+  /* get_inu, src/pixelize.c:28-55 (table mode); the uniform (int) cast of :216 otherwise */
   if (d->nu_mode == CRIME_GH_NU_UNIFORM) {
     double inv_dnu = d->n_nu / (d->nu_max - d->nu_min);
     double t = inv_dnu * (nu - d->nu_min);
@@ -442,7 +442,7 @@ int orc_get_inu(const crime_gh_desc *d, double nu, int inu_start)
 
 void orc_channel_factors(const crime_gh_desc *d, double *prefac)
 {
-  /* src/pixelize.c:308, 399-409 */
+  /* src/pixelize.c:155, 246-256 */
   const double m2t = 90.057156 * d->omega_b * d->hhub * (double)(12 * d->n_side * d->n_side) / (4 * M_PI);
   for (int inu = 0; inu < d->n_nu; inu++) {
     double dnu, nu;
@@ -512,7 +512,7 @@ int orc_mk_T_maps(const crime_gh_desc *d, const double *mass, const double *dzrs
     int64_t *pix_s = (int64_t *)malloc(sizeof(int64_t) * d->n_sub);
 #pragma omp for schedule(static)
     for (int iz = 0; iz < n; iz++) {
-      int state = 0; /* the walk restarts at channel 0 on every z-plane (pixelize.c:345) */
+      int state = 0; /* the walk restarts at channel 0 on every z-plane (pixelize.c:192) */
       for (int iy = 0; iy < n; iy++) {
         for (int ix = 0; ix < n; ix++) {
           size_t index = ix + iy * nn + iz * n2;

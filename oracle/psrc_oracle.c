@@ -1,8 +1,8 @@
 /* ORACLE -- TEST INFRASTRUCTURE ONLY.  FP64 restatement of the reference's point sources:
- *   setup_psources      src/psources.c:112-145    (n(z) and max_Lpdf tables)
- *   n_of_z_psources     src/psources.c:78-95
- *   draw_luminosity     src/psources.c:147-170    (rejection sampling of log10 L)
- *   temp_of_l / spec_ed src/psources.c:54-63, 173-184
+ *   setup_psources      src/psources.c:99-128    (n(z) and max_Lpdf tables)
+ *   n_of_z_psources     src/psources.c:74-90
+ *   draw_luminosity     src/psources.c:130-152    (rejection sampling of log10 L)
+ *   temp_of_l / spec_ed src/psources.c:54-64, 157-167
  *   get_point_sources   src/grid_tools.c:24-101   (Poisson count per cell; x offset by pos_obs[1], :66)
  *   mk_psources_maps    src/pixelize.c:58-148     (uniform position in the cell, every channel)
  * The random numbers follow the GPU library's documented streams (Philox-4x32-10, key (seed,
@@ -51,14 +51,14 @@ double orc_ps_lfunc(double L, double z)
   return fz * rho_l;
 }
 
-/* spec_ed (src/psources.c:54-63) */
+/* spec_ed (src/psources.c:54-64) */
 double orc_ps_sed(double nu)
 {
   double nu_ghz = nu * 0.001;
   return SED_NORM * (pow(nu_ghz, -0.1) + 10 * pow(nu_ghz, -0.75));
 }
 
-/* temp_of_l (src/psources.c:173-184): mK of a source of luminosity L (1e22 W/Hz/sr) at nu_obs */
+/* temp_of_l (src/psources.c:157-167): mK of a source of luminosity L (1e22 W/Hz/sr) at nu_obs */
 double orc_ps_temp(double hhub, double L, double nu_obs, double z, double r, double domega)
 {
   double s_nu = LUM2FLUX * 4 * M_PI * L * orc_ps_sed((1 + z) * nu_obs) * hhub * hhub / (r * r * (1 + z));
@@ -80,7 +80,7 @@ void orc_ps_tables(double z_max, double *nz, double *maxpdf)
   }
 }
 
-/* n_of_z_psources (src/psources.c:78-95), with glob_inv_dz = 1 / glob_dz (:118) */
+/* n_of_z_psources (src/psources.c:74-90), with glob_inv_dz = 1 / glob_dz (:107) */
 static double nofz(const double *nz, double dz, double z)
 {
   const double inv_dz = 1. / dz;

@@ -60,7 +60,7 @@ def test_reference_style_parameter_file(tmp_path):
     assert par.get("n_nu") == 150            # 151 edges (src/io_gh.c:36)
     assert par.get("nu_min") == 355.0 and par.get("nu_max") == 945.0
     assert par.get("n_side") == 512 and par.get("n_grid") == 64 and par.get("seed") == 1001
-    assert par.get("n_sub") == 10            # N_SUBPART (src/pixelize.c:178)
+    assert par.get("n_sub") == 10            # N_SUBPART (src/pixelize.c:25)
     d = par.desc()
     assert d.numk == 807
     assert abs(d.r2_smooth - 4.0) < 1e-15 and d.do_smoothing == 1     # squared (io_gh.c:255-260)
@@ -156,7 +156,7 @@ def test_subcell_offsets():
     n = H.gh_subcell_offsets(0, lcell, 2002, out.ctypes.data_as(C.POINTER(C.c_double)))
     assert n == 10
     u = np.random.RandomState(2002).randint(0, 2 ** 32, size=30, dtype=np.uint32) / 2.0 ** 32
-    assert np.array_equal(out, lcell * (u - 0.5))   # x,y,z per sample (src/pixelize.c:312-316)
+    assert np.array_equal(out, lcell * (u - 0.5))   # x,y,z per sample (src/pixelize.c:159-163)
     out = np.empty(3 * 27)
     assert H.gh_subcell_offsets(3, lcell, 0, out.ctypes.data_as(C.POINTER(C.c_double))) == 27
     o = out.reshape(27, 3)

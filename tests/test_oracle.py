@@ -239,7 +239,7 @@ def test_maps_mass_conservation_and_factor(cfgdir):
     ok = inu >= 0
     per_ch = np.bincount(inu[ok], weights=msub[ok], minlength=d.n_nu)
     assert np.allclose((maps / pref[:, None]).sum(axis=1), per_ch, rtol=1e-12, atol=0)
-    # mK factor 90.057156 Omega_b h Npix / (4 pi) / (r(nu_c)^2 dnu) (pixelize.c:308, 399-409)
+    # mK factor 90.057156 Omega_b h Npix / (4 pi) / (r(nu_c)^2 dnu) (pixelize.c:155, 246-256)
     nu0 = np.ctypeslib.as_array(d.nu0, shape=(d.n_nu,))
     nuf = np.ctypeslib.as_array(d.nuf, shape=(d.n_nu,))
     npix = 12 * d.n_side ** 2
