@@ -18,11 +18,13 @@ import numpy as np
 
 from . import _ffi
 from ._ffi import (BIAS_CONST, BIAS_LINEAR, BIAS_REFERENCE, FP32, FP64, GRID_DENS, GRID_DZRSD, GRID_MASS,
-                   GRID_RVEL, GRID_VPOT, MODES_INJECTED, MODES_PHILOX, NU_TABLE, NU_UNIFORM, CrimeGhDesc)
+                   GRID_RVEL, GRID_VPOT, GRID_VX, GRID_VY, GRID_VZ, MODES_INJECTED, MODES_PHILOX, NU_TABLE,
+                   NU_UNIFORM, VEL_SPECTRAL, VEL_STENCIL, CrimeGhDesc)
 
 __all__ = ["Params", "GetHI", "Sht", "Batch", "run_batch", "alm_index", "jt_instrument", "CrimeError", "write_ini",
            "write_bbks_pk", "FP32", "FP64",
-           "GRID_DENS", "GRID_VPOT", "GRID_RVEL", "GRID_MASS", "GRID_DZRSD", "MODES_PHILOX",
+           "GRID_DENS", "GRID_VPOT", "GRID_RVEL", "GRID_MASS", "GRID_DZRSD", "GRID_VX", "GRID_VY", "GRID_VZ",
+           "VEL_STENCIL", "VEL_SPECTRAL", "MODES_PHILOX",
            "MODES_INJECTED", "NU_TABLE", "NU_UNIFORM", "BIAS_REFERENCE", "BIAS_CONST", "BIAS_LINEAR"]
 
 
@@ -147,6 +149,29 @@ class GetHI:
     def grid(self, which):
         out = np.empty((self.n, self.n, self.n), np.float64)
         self._check(self._lib.crime_gh_copy_grid(self._h, which, _dptr(out)))
+        return out
+
+    def grid_range(self, which, cell0, ncell):
+        """Cells [cell0, cell0 + ncell) of a grid (x fastest), as FP64 (crime_gh_copy_grid_range)."""
+        out = np.empty(ncell, np.float64)
+        self._check(self._lib.crime_gh_copy_grid_range(self._h, which, int(cell0), int(ncell), _dptr(out)))
+        return out
+
+    def channel_totals(self, cell0=0, ncell=None):
+        """FP64 per-channel sums of mass/n_sub x mK factor over every sample of cells [cell0, cell0+ncell),
+        and the number of binned samples per channel (crime_gh_channel_totals)."""
+        ncell = self.n ** 3 - cell0 if ncell is None else ncell
+        tot = np.empty(self.n_nu)
+        cnt = np.empty(self.n_nu, np.int64)
+        self._check(self._lib.crime_gh_channel_totals(self._h, int(cell0), int(ncell), _dptr(tot),
+                                                      cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+        return tot, cnt
+
+    def reference_maps(self):
+        """FP64 maps scattered sample by sample without the cell screen or the LDS aggregation
+        (crime_gh_reference_maps): the parity target of the timed map kernel."""
+        out = np.empty((self.n_nu, self.npix), np.float64)
+        self._check(self._lib.crime_gh_reference_maps(self._h, _dptr(out)))
         return out
 
     def set_grid(self, which, arr):

@@ -11,13 +11,15 @@ import os
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 NZ = 5001
 FP32, FP64 = 0, 1
 MODES_PHILOX, MODES_INJECTED = 0, 1
 NU_TABLE, NU_UNIFORM = 0, 1
 BIAS_REFERENCE, BIAS_CONST, BIAS_LINEAR = 0, 1, 2
 GRID_DENS, GRID_VPOT, GRID_RVEL, GRID_MASS, GRID_DZRSD = 0, 1, 2, 3, 4
+GRID_VX, GRID_VY, GRID_VZ = 5, 6, 7
+VEL_STENCIL, VEL_SPECTRAL = 0, 1
 
 ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ENODEV", -5: "ESTATE", -6: "EABI"}
 
@@ -70,6 +72,8 @@ class CrimeGhDesc(C.Structure):
         ("mode_source", C.c_int32),
         ("unit_modes", _dp),
         ("device", C.c_int32),
+        ("velocity_mode", C.c_int32),
+        ("deterministic", C.c_int32),
     ]
 
 
@@ -165,6 +169,12 @@ def gpu_lib():
         lib.crime_gh_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), _dp]
         lib.crime_gh_power_spectrum.restype = C.c_int
         lib.crime_gh_power_spectrum.argtypes = [H, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int64)]
+        lib.crime_gh_copy_grid_range.restype = C.c_int
+        lib.crime_gh_copy_grid_range.argtypes = [H, C.c_int, C.c_int64, C.c_int64, _dp]
+        lib.crime_gh_channel_totals.restype = C.c_int
+        lib.crime_gh_channel_totals.argtypes = [H, C.c_int64, C.c_int64, _dp, C.POINTER(C.c_int64)]
+        lib.crime_gh_reference_maps.restype = C.c_int
+        lib.crime_gh_reference_maps.argtypes = [H, _dp]
         lib.crime_gh_device_maps.restype = C.c_int
         lib.crime_gh_device_maps.argtypes = [H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
         lib.crime_gh_mk_psources_maps.restype = C.c_int

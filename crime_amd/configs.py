@@ -21,9 +21,11 @@ def _edges(path, nu_lo, nu_hi, n_nu):
 
 def make(outdir, name, *, n_grid, n_side, n_nu, nu_lo, nu_hi, l_box, seed, omega_m=0.3, omega_b=0.05,
          h=0.7, sigma_8=0.8, n_s=0.96, precision="single", bias_model="reference", bias_b0=1.0,
-         bias_b1=0.0, n_subcell=0, rsd=1, modes="philox", r_smooth=-1.0, uniform_nu=False, extra=None):
+         bias_b1=0.0, n_subcell=0, rsd=1, modes="philox", r_smooth=-1.0, uniform_nu=False,
+         velocity_mode="stencil", extra=None):
     """uniform_nu: the reference's non-_IRREGULAR_NUTABLE binning (nu_min/nu_max/n_nu keys) instead
-    of a table of edges."""
+    of a table of edges.  velocity_mode: "stencil" (the reference's central difference of phi) or
+    "spectral" (three velocity-component FFTs)."""
     os.makedirs(outdir, exist_ok=True)
     pk = os.path.join(outdir, f"{name}_pk.dat")
     write_bbks_pk(pk, omega_m, omega_b, h, n_s)
@@ -39,6 +41,8 @@ def make(outdir, name, *, n_grid, n_side, n_nu, nu_lo, nu_hi, l_box, seed, omega
     keys.update(n_side=n_side, n_grid=n_grid, seed=seed, do_psources=0, l_box=l_box,
                 bias_model=bias_model, bias_b0=bias_b0, bias_b1=bias_b1, n_subcell=n_subcell,
                 precision=precision, rsd=rsd, modes=modes)
+    if velocity_mode != "stencil":
+        keys.update(velocity_mode=velocity_mode)
     if extra:
         keys.update(extra)
     return write_ini(os.path.join(outdir, f"{name}.ini"), **keys)
@@ -66,9 +70,9 @@ def production_1024(outdir, **kw):
 
 def largest_2048(outdir, **kw):
     """FP32 grids, 2048^3, L=8300 Mpc/h, 400 channels 400-800 MHz, Nside 1024, 4^3 sub-cells,
-    sigma_8=0.83, n_s=0.965, seed 3003."""
+    three velocity components (spectral velocities), sigma_8=0.83, n_s=0.965, seed 3003."""
     a = dict(n_grid=2048, n_side=1024, n_nu=400, nu_lo=400.0, nu_hi=800.0, l_box=8300.0, seed=3003,
-             sigma_8=0.83, n_s=0.965, precision="single", n_subcell=4)
+             sigma_8=0.83, n_s=0.965, precision="single", n_subcell=4, velocity_mode="spectral")
     name = kw.pop("name", "largest_2048")
     a.update(kw)
     return make(outdir, name, **a)

@@ -21,6 +21,10 @@ struct crime_gh {
   int64_t nside = 0, npix = 0;
   hipStream_t stream = nullptr;
   void *F = nullptr;
+  void *G = nullptr;           // (vy, vz) of CRIME_GH_VEL_SPECTRAL handles
+  double *d_sin = nullptr;     // [N] sin(2 pi m / N) / dx of the spectral velocities
+  unsigned long long *d_fx = nullptr;  // deterministic scatter: integer maps [n_nu * npix]
+  double *d_fxs = nullptr;     // [2 + FX_PARTS]: s, 1/s, the mass partials
   size_t ncell = 0;
   void *maps = nullptr;
   size_t nmaps = 0;
@@ -53,6 +57,7 @@ struct crime_gh {
 };
 
 static thread_local std::string g_create_err;
+
 
 static int set_error(crime_gh *h, int code, const char *fmt, ...)
 {
@@ -104,11 +109,12 @@ static double pk_host(const crime_gh_desc *d, double lgk)
   return d->pk[d->numk - 1] * pow(10, -3 * (lgk - d->logkmax));
 }
 
-// rms(delta)/rms(phi) of the grid's modes, estimated with a spherical k-space integral from the
-// fundamental to the radius of the sphere with the grid cube's volume.  Only its order of magnitude
-// matters: phi is packed as phi_scale*phi next to delta in one complex FP32 element, so that both
-// fields carry rounding errors relative to their own size.
-static double phi_balance(const crime_gh_desc *d)
+// rms(delta)/rms(phi) of the grid's modes (rms(delta)/rms(v_j) for spectral velocities), estimated
+// with a spherical k-space integral from the fundamental to the radius of the sphere with the grid
+// cube's volume.  Only its order of magnitude matters: phi (or v_j) is packed as phi_scale*phi next to
+// delta in one complex FP32 element, so that both fields carry rounding errors relative to their own
+// size.
+static double phi_balance(const crime_gh_desc *d, bool velocity)
 {
   double dk = 2 * M_PI / d->l_box;
   double kmax = 0.5 * d->n_grid * dk * cbrt(6 / M_PI);
@@ -118,7 +124,8 @@ static double phi_balance(const crime_gh_desc *d)
     double k = exp(l0 + (l1 - l0) * (i + 0.5) / nk);
     double p = pk_host(d, log10(k)) * k * k * k;  // k^2 P(k) dk with dk = k dlnk
     if (d->do_smoothing) p *= exp(-d->r2_smooth * k * k);
-    double psi = d->f0h0 / (k * k);
+    // phi = f0 H0 / k^2 delta; one velocity component ~ f0 H0 / k delta, k_j^2 ~ k^2 / 3 on average
+    double psi = velocity ? d->f0h0 / (k * sqrt(3.0)) : d->f0h0 / (k * k);
     i0 += p;
     i2 += p * psi * psi;
   }
@@ -166,6 +173,10 @@ static int validate(const crime_gh_desc *d)
   if (d->mode_source == CRIME_GH_MODES_INJECTED && !d->unit_modes)
     return set_error(nullptr, CRIME_GH_EINVAL, "unit_modes missing for injected modes");
   if (d->bias_model < 0 || d->bias_model > 2) return set_error(nullptr, CRIME_GH_EINVAL, "bias_model %d", d->bias_model);
+  if (d->velocity_mode != CRIME_GH_VEL_STENCIL && d->velocity_mode != CRIME_GH_VEL_SPECTRAL)
+    return set_error(nullptr, CRIME_GH_EINVAL, "velocity_mode %d", d->velocity_mode);
+  if (d->deterministic != 0 && d->deterministic != 1)
+    return set_error(nullptr, CRIME_GH_EINVAL, "deterministic %d", d->deterministic);
   return 0;
 }
 
@@ -182,7 +193,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub,
+  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig, h->d_ps, h->d_ps_count, h->ps_maps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -235,7 +246,24 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   for (auto &e : h->ev) HIP_OR_FAIL(h, hipEventCreate(&e));
 
   HIP_OR_FAIL(h, hipMalloc(&h->F, h->ncell * 2 * esz));
+  if (d->velocity_mode == CRIME_GH_VEL_SPECTRAL) {
+    HIP_OR_FAIL(h, hipMalloc(&h->G, h->ncell * 2 * esz));
+    // sin(k_j dx)/dx at k_j dx = 2 pi m / N: exactly 0 at m = 0 and N/2, odd under m -> N - m
+    std::vector<double> sn(h->n, 0.0);
+    const double dx = d->l_box / h->n;
+    for (int m = 1; 2 * m < h->n; m++) {
+      const double v = (double)(sinl(2.0L * 3.14159265358979323846264338327950288L * m / h->n) / (long double)dx);
+      sn[m] = v;
+      sn[h->n - m] = -v;
+    }
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_sin, h->n * sizeof(double)));
+    HIP_OR_FAIL(h, hipMemcpy(h->d_sin, sn.data(), h->n * sizeof(double), hipMemcpyHostToDevice));
+  }
   HIP_OR_FAIL(h, hipMalloc(&h->maps, h->nmaps * esz));
+  if (d->deterministic) {
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_fx, h->nmaps * sizeof(unsigned long long)));
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_fxs, (2 + FX_PARTS) * sizeof(double)));
+  }
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_tab, 4 * CRIME_GH_NZ * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_pk, 2 * (size_t)d->numk * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_edges, (d->n_nu + 1) * sizeof(double)));
@@ -344,7 +372,10 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   f.mode_source = d->mode_source;
   f.unit_modes = h->d_modes;
   h->norm = pow(sqrt(2 * M_PI) / d->l_box, 3);  // fourier.c:403
-  f.phi_scale = phi_balance(d);
+  f.vmode = d->velocity_mode;
+  f.comp = 0;
+  f.sin_dx = h->d_sin;
+  f.phi_scale = phi_balance(d, d->velocity_mode == CRIME_GH_VEL_SPECTRAL);
   f.psi_c = f.phi_scale * d->f0h0 / (f.dk * f.dk);
   HIP_OR_FAIL(h, hipMalloc(&h->d_amp, (size_t)amp_count(h->n) * 2 * esz));
   f.amp = h->d_amp;
@@ -427,6 +458,12 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   m.nside = h->nside;
   m.npix = h->npix;
   m.sig = h->d_sig;
+  m.vmode = d->velocity_mode;
+  m.G = h->G;
+  m.fx = d->deterministic ? 1 : 0;
+  m.fxmaps = h->d_fx;
+  m.fxs = h->d_fxs;
+  m.mass_part = h->d_fxs ? h->d_fxs + 2 : nullptr;
   double zero2[2] = {0, 0};
   HIP_OR_FAIL(h, hipMemcpy(h->d_sig, zero2, sizeof(zero2), hipMemcpyHostToDevice));
   return 0;
@@ -462,10 +499,10 @@ static int enqueue_fields(crime_gh *h, bool timed, hipStream_t s = nullptr)
   hipError_t e;
   if (!s) s = h->stream;
   if (h->prec)
-    e = launch_fields<double>(h->n, (cx<double> *)h->F, h->fill, (const cx<double> *)h->d_tw, h->norm,
+    e = launch_fields<double>(h->n, (cx<double> *)h->F, (cx<double> *)h->G, h->fill, (const cx<double> *)h->d_tw, h->norm,
                               h->d_partials, h->d_sig, s, timed ? h->ev : nullptr, h->sched);
   else
-    e = launch_fields<float>(h->n, (cx<float> *)h->F, h->fill, (const cx<float> *)h->d_tw, h->norm,
+    e = launch_fields<float>(h->n, (cx<float> *)h->F, (cx<float> *)h->G, h->fill, (const cx<float> *)h->d_tw, h->norm,
                              h->d_partials, h->d_sig, s, timed ? h->ev : nullptr, h->sched);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "field kernels: %s", hipGetErrorString(e));
   return 0;
@@ -577,6 +614,12 @@ extern "C" int crime_gh_power_spectrum(crime_gh_t *h, int nbins, double bin_dk, 
     pk[i] = c > 0 ? hacc[3 * i] / c : 0.0;
   }
   return 0;
+}
+
+// zero n doubles (a kernel, not a memset: no memset node if a caller captures the stream)
+__global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ p, size_t n)
+{
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0;
 }
 
 extern "C" int crime_gh_get_schedule(const crime_gh_t *h, int *sched)
@@ -711,39 +754,71 @@ extern "C" int crime_gh_set_sigma2(crime_gh_t *h, double sigma2, double mean)
   return 0;
 }
 
-extern "C" int crime_gh_copy_grid(crime_gh_t *h, int which, double *dst)
+// grid holding a stored field and its half (0 = real, 1 = imaginary); false for a derived field
+static bool stored_grid(crime_gh *h, int which, void **grid, int *half)
+{
+  const bool spec = h->d.velocity_mode == CRIME_GH_VEL_SPECTRAL;
+  switch (which) {
+  case CRIME_GH_GRID_DENS: *grid = h->F; *half = 0; return true;
+  case CRIME_GH_GRID_VPOT: *grid = spec ? nullptr : h->F; *half = 1; return true;
+  case CRIME_GH_GRID_VX: *grid = spec ? h->F : nullptr; *half = 1; return true;
+  case CRIME_GH_GRID_VY: *grid = h->G; *half = 0; return true;
+  case CRIME_GH_GRID_VZ: *grid = h->G; *half = 1; return true;
+  default: return false;
+  }
+}
+
+extern "C" int crime_gh_copy_grid_range(crime_gh_t *h, int which, int64_t cell0, int64_t ncell, double *dst)
 {
   if (!h || !dst) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
-  if (which < CRIME_GH_GRID_DENS || which > CRIME_GH_GRID_DZRSD) return set_error(h, CRIME_GH_EINVAL, "grid %d", which);
+  if (which < CRIME_GH_GRID_DENS || which > CRIME_GH_GRID_VZ) return set_error(h, CRIME_GH_EINVAL, "grid %d", which);
+  if (cell0 < 0 || ncell < 1 || (size_t)(cell0 + ncell) > h->ncell)
+    return set_error(h, CRIME_GH_EINVAL, "cell range [%lld,%lld) outside the grid", (long long)cell0,
+                     (long long)(cell0 + ncell));
+  void *grid = nullptr;
+  int half = 0;
+  const bool stored = stored_grid(h, which, &grid, &half);
+  if (stored && !grid)
+    return set_error(h, CRIME_GH_EINVAL, "grid %d is not held in velocity mode %d", which, h->d.velocity_mode);
   HIP_OR_FAIL(h, hipSetDevice(h->d.device));
   double *tmp = nullptr;
-  HIP_OR_FAIL(h, hipMalloc((void **)&tmp, h->ncell * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&tmp, (size_t)ncell * sizeof(double)));
   hipError_t e;
-  if (which <= CRIME_GH_GRID_VPOT)
-    e = h->prec ? launch_grid_get<double>((const cx<double> *)h->F, h->ncell, which, tmp, h->stream)
-                : launch_grid_get<float>((const cx<float> *)h->F, h->ncell, which, tmp, h->stream);
+  const size_t esz = h->prec ? sizeof(cx<double>) : sizeof(cx<float>);
+  if (stored)
+    e = h->prec ? launch_grid_get<double>((const cx<double> *)((char *)grid + cell0 * esz), ncell, half, tmp, h->stream)
+                : launch_grid_get<float>((const cx<float> *)((char *)grid + cell0 * esz), ncell, half, tmp, h->stream);
   else
-    e = h->prec ? launch_cell_field<double>((const cx<double> *)h->F, h->map, which, tmp, h->stream)
-                : launch_cell_field<float>((const cx<float> *)h->F, h->map, which, tmp, h->stream);
+    e = h->prec ? launch_cell_field<double>((const cx<double> *)h->F, h->map, which, cell0, ncell, tmp, h->stream)
+                : launch_cell_field<float>((const cx<float> *)h->F, h->map, which, cell0, ncell, tmp, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  if (e == hipSuccess) e = hipMemcpy(dst, tmp, h->ncell * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(dst, tmp, (size_t)ncell * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(tmp);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "copy_grid: %s", hipGetErrorString(e));
   return 0;
 }
 
+extern "C" int crime_gh_copy_grid(crime_gh_t *h, int which, double *dst)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  return crime_gh_copy_grid_range(h, which, 0, (int64_t)h->ncell, dst);
+}
+
 extern "C" int crime_gh_set_grid(crime_gh_t *h, int which, const double *src)
 {
   if (!h || !src) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
-  if (which != CRIME_GH_GRID_DENS && which != CRIME_GH_GRID_VPOT)
-    return set_error(h, CRIME_GH_EINVAL, "only DENS and VPOT can be set");
+  void *grid = nullptr;
+  int half = 0;
+  if (which < CRIME_GH_GRID_DENS || which > CRIME_GH_GRID_VZ || !stored_grid(h, which, &grid, &half) || !grid)
+    return set_error(h, CRIME_GH_EINVAL, "grid %d cannot be set (DENS and VPOT, or DENS and VX/VY/VZ in spectral mode)",
+                     which);
   HIP_OR_FAIL(h, hipSetDevice(h->d.device));
   double *tmp = nullptr;
   HIP_OR_FAIL(h, hipMalloc((void **)&tmp, h->ncell * sizeof(double)));
   hipError_t e = hipMemcpy(tmp, src, h->ncell * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = h->prec ? launch_grid_set<double>((cx<double> *)h->F, h->ncell, which, tmp, h->stream)
-                : launch_grid_set<float>((cx<float> *)h->F, h->ncell, which, tmp, h->stream);
+    e = h->prec ? launch_grid_set<double>((cx<double> *)grid, h->ncell, half, tmp, h->stream)
+                : launch_grid_set<float>((cx<float> *)grid, h->ncell, half, tmp, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "set_grid: %s", hipGetErrorString(e));
@@ -772,6 +847,55 @@ extern "C" int crime_gh_sample_indices(crime_gh_t *h, int64_t cell0, int64_t nce
   (void)hipFree(d_inu);
   if (d_pix) (void)hipFree(d_pix);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "sample_indices: %s", hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" int crime_gh_channel_totals(crime_gh_t *h, int64_t cell0, int64_t ncell, double *tot, int64_t *count)
+{
+  if (!h || !tot) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (cell0 < 0 || ncell < 1 || (size_t)(cell0 + ncell) > h->ncell)
+    return set_error(h, CRIME_GH_EINVAL, "cell range [%lld,%lld) outside the grid", (long long)cell0,
+                     (long long)(cell0 + ncell));
+  if (h->n_nu > 2048) return set_error(h, CRIME_GH_EINVAL, "channel_totals supports n_nu <= 2048");
+  if (!h->fields_done) return set_error(h, CRIME_GH_ESTATE, "channel_totals before create_d_and_vr_fields");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  double *d_tot = nullptr;
+  HIP_OR_FAIL(h, hipMalloc((void **)&d_tot, (size_t)h->n_nu * (sizeof(double) + sizeof(unsigned long long))));
+  unsigned long long *d_cnt = (unsigned long long *)(d_tot + h->n_nu);
+  hipError_t e = h->prec ? launch_channel_totals<double>((const cx<double> *)h->F, h->map, cell0, ncell, d_tot, d_cnt,
+                                                         h->stream)
+                         : launch_channel_totals<float>((const cx<float> *)h->F, h->map, cell0, ncell, d_tot, d_cnt,
+                                                        h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(tot, d_tot, h->n_nu * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && count) e = hipMemcpy(count, d_cnt, h->n_nu * sizeof(int64_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d_tot);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(h, CRIME_GH_EHIP, "channel_totals: %s", hipGetErrorString(e));
+  }
+  return 0;
+}
+
+extern "C" int crime_gh_reference_maps(crime_gh_t *h, double *out)
+{
+  if (!h || !out) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (!h->fields_done) return set_error(h, CRIME_GH_ESTATE, "reference_maps before create_d_and_vr_fields");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  double *ref = nullptr;
+  HIP_OR_FAIL(h, hipMalloc((void **)&ref, h->nmaps * sizeof(double)));
+  hipLaunchKernelGGL(k_zero_f64, dim3(4096), dim3(256), 0, h->stream, ref, h->nmaps);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = h->prec ? launch_reference_maps<double>((const cx<double> *)h->F, h->map, ref, h->stream)
+                : launch_reference_maps<float>((const cx<float> *)h->F, h->map, ref, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, ref, h->nmaps * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(ref);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(h, CRIME_GH_EHIP, "reference_maps: %s", hipGetErrorString(e));
+  }
   return 0;
 }
 

@@ -101,6 +101,11 @@ struct FillParams {
   const double *unit_modes;
   const void *amp;  // AmpEntry<T>[amp_count(n)] for k^2 = m dk^2, filled by launch_amp_table
   double psi_c;     // phi_scale f0 H0 / dk^2: psi = psi_c / m
+  // velocity mode (CRIME_GH_VEL_*).  SPECTRAL: phi_scale scales the velocity components instead of
+  // phi, and grid `comp` 0 holds delta + i phi_scale vx, grid 1 phi_scale (vy + i vz), with
+  // v_j(k) = i sin_dx[m_j] phi_k and sin_dx[m] = sin(2 pi m / N) / dx (exactly 0 at m = 0, N/2)
+  int vmode, comp;
+  const double *sin_dx;
 };
 
 // amplitude-table entry: A (FP32 grids) or (A, A psi) (FP64 grids)
@@ -136,7 +141,17 @@ struct MapParams {
   double g_lo, g_inv;
   const uint32_t *tile_order;  // tiles by increasing distance of their centre from the observer
   int ntx, nty, ntz;
+  int vmode;               // CRIME_GH_VEL_*: SPECTRAL reads vx from F.y and (vy, vz) from G
+  const void *G;           // cx<T>[N^3] second grid of SPECTRAL handles (NULL otherwise)
+  // deterministic scatter: integer mass sums in fx[n_nu * npix], scale s and 1/s in fxs[0..1],
+  // FX_PARTS fixed-order partial sums of the box's mass in mass_part
+  int fx;
+  unsigned long long *fxmaps;
+  double *fxs;
+  double *mass_part;
 };
+
+constexpr int FX_PARTS = 2048;  // blocks of the fixed-order mass reduction
 
 // LDS aggregation of the scatter: one workgroup per TILE_X x TILE_Y x TILE_Z cells, AGG_SLOTS bins.
 // TILE_X = 16 cells of complex<float> = one 128-byte line per row of the grid.
@@ -170,8 +185,10 @@ template <typename T>
 hipError_t launch_ps_expand(const double *A, const double *B, const double *c1, const double *c2, int n_nu,
                             int64_t npix, T *maps, hipStream_t s);
 
+// Field stage.  STENCIL: F = delta + i phi.  SPECTRAL: G = (vy, vz) first, then F = delta + i vx (the
+// sigma^2 partials are those of F, whose kx pass runs last).
 template <typename T>
-hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
+hipError_t launch_fields(int n, cx<T> *F, cx<T> *G, const FillParams &f, const cx<T> *tw, double norm,
                          double *partials, double *sig, hipStream_t s, hipEvent_t *ev, const Schedule &sc);
 
 constexpr int PK_MAX_BINS = 2048;
@@ -184,10 +201,18 @@ template <typename T>
 hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m, hipStream_t s,
                        hipEvent_t *ev, const Schedule &sc);
 template <typename T>
-hipError_t launch_cell_field(const cx<T> *F, const MapParams &m, int which, double *out, hipStream_t s);
+hipError_t launch_cell_field(const cx<T> *F, const MapParams &m, int which, size_t cell0, size_t ncell, double *out,
+                             hipStream_t s);
 template <typename T>
 hipError_t launch_sample_indices(const cx<T> *F, const MapParams &m, int64_t cell0, int64_t ncell,
                                  int32_t *inu, int64_t *ipix, hipStream_t s);
+// FP64 parity scatter of every sample: per-channel sums (tot[n_nu], cnt[n_nu]; cleared here) or whole maps
+// (ref[n_nu * npix], cleared by the caller)
+template <typename T>
+hipError_t launch_channel_totals(const cx<T> *F, const MapParams &m, int64_t cell0, int64_t ncell, double *tot,
+                                 unsigned long long *cnt, hipStream_t s);
+template <typename T>
+hipError_t launch_reference_maps(const cx<T> *F, const MapParams &m, double *ref, hipStream_t s);
 template <typename T>
 hipError_t launch_grid_get(const cx<T> *F, size_t ncell, int which, double *out, hipStream_t s);
 template <typename T>

@@ -122,8 +122,20 @@ __device__ __forceinline__ cx<T> fill_element(const FillParams &f, const AmpEntr
     cx<T> a = unit_mode<T>(f, kx, ky, kz), b = unit_mode<T>(f, kx, myn, mzn);
     g = {(T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)};
   }
-  // delta = A g, phi_scale phi = A psi g; F = delta + i phi_scale phi
-  return {ab.x * g.x - ab.y * g.y, ab.x * g.y + ab.y * g.x};
+  if (f.vmode == 0) {
+    // delta = A g, phi_scale phi = A psi g; F = delta + i phi_scale phi = A g (1 + i psi)
+    return {ab.x * g.x - ab.y * g.y, ab.x * g.y + ab.y * g.x};
+  }
+  // spectral velocities: with A psi = A phi_scale f0 H0 / k^2 and S_j = sin(k_j dx)/dx,
+  //   grid 0: delta + i phi_scale vx = A g + i (i S_x A psi g) = (A - S_x A psi) g
+  //   grid 1: phi_scale (vy + i vz)  = i S_y A psi g - S_z A psi g = A psi (-S_z + i S_y) g
+  // S_j is odd in k_j, so each v_j(k) is Hermitian wherever delta_k is (the kx = 0, N/2 planes included).
+  if (f.comp == 0) {
+    const T c = ab.x - (T)f.sin_dx[kx] * ab.y;
+    return {c * g.x, c * g.y};
+  }
+  const cx<T> c = {-(T)f.sin_dx[kz] * ab.y, (T)f.sin_dx[ky] * ab.y};
+  return {c.x * g.x - c.y * g.y, c.x * g.y + c.y * g.x};
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -359,25 +371,53 @@ static int launch_x_v(cx<T> *F, const cx<T> *tw, double norm, double norm_phi, d
 }
 
 template <typename T, int N>
-static hipError_t launch_fields_n(cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
-                                  double *partials, double *sig, hipStream_t s, hipEvent_t *ev,
-                                  const Schedule &sc)
+static void fill_sched(cx<T> *F, const FillParams &f, const cx<T> *tw, hipStream_t s, const Schedule &sc)
 {
   constexpr bool tuned = N >= TUNE_MIN_N;
-  if (ev) (void)hipEventRecord(ev[0], s);
   if (tuned && sc.fill == 1) launch_fill_v<T, N, tuned ? 1 : 0>(F, f, tw, s);
   else if (tuned && sc.fill == 2) launch_fill_v<T, N, tuned ? 2 : 0>(F, f, tw, s);
   else launch_fill_v<T, N, 0>(F, f, tw, s);
-  if (ev) (void)hipEventRecord(ev[1], s);
+}
+
+template <typename T, int N>
+static void y_sched(cx<T> *F, const cx<T> *tw, hipStream_t s, const Schedule &sc)
+{
+  constexpr bool tuned = N >= TUNE_MIN_N;
   if (tuned && sc.y == 1) launch_y_v<T, N, tuned ? 1 : 0>(F, tw, s);
   else if (tuned && sc.y == 2) launch_y_v<T, N, tuned ? 2 : 0>(F, tw, s);
   else launch_y_v<T, N, 0>(F, tw, s);
+}
+
+template <typename T, int N>
+static int x_sched(cx<T> *F, const cx<T> *tw, double nre, double nim, double *partials, hipStream_t s,
+                   const Schedule &sc)
+{
+  constexpr bool tuned = N >= TUNE_MIN_N;
+  if (tuned && sc.x == 1) return launch_x_v<T, N, tuned ? 1 : 0>(F, tw, nre, nim, partials, s);
+  if (tuned && sc.x == 2) return launch_x_v<T, N, tuned ? 2 : 0>(F, tw, nre, nim, partials, s);
+  return launch_x_v<T, N, 0>(F, tw, nre, nim, partials, s);
+}
+
+// Stage timings (ev[0..4]) cover both grids of a spectral handle: the second grid's pass runs right
+// before the same pass of F, and its sigma^2 partials are overwritten by F's kx pass.
+template <typename T, int N>
+static hipError_t launch_fields_n(cx<T> *F, cx<T> *G, const FillParams &f, const cx<T> *tw, double norm,
+                                  double *partials, double *sig, hipStream_t s, hipEvent_t *ev,
+                                  const Schedule &sc)
+{
+  const bool two = f.vmode != 0;
+  FillParams f1 = f;
+  f1.comp = 1;
+  if (ev) (void)hipEventRecord(ev[0], s);
+  if (two) fill_sched<T, N>(G, f1, tw, s, sc);
+  fill_sched<T, N>(F, f, tw, s, sc);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  if (two) y_sched<T, N>(G, tw, s, sc);
+  y_sched<T, N>(F, tw, s, sc);
   if (ev) (void)hipEventRecord(ev[2], s);
   const double nphi = norm / f.phi_scale;
-  int nrowblk;
-  if (tuned && sc.x == 1) nrowblk = launch_x_v<T, N, tuned ? 1 : 0>(F, tw, norm, nphi, partials, s);
-  else if (tuned && sc.x == 2) nrowblk = launch_x_v<T, N, tuned ? 2 : 0>(F, tw, norm, nphi, partials, s);
-  else nrowblk = launch_x_v<T, N, 0>(F, tw, norm, nphi, partials, s);
+  if (two) x_sched<T, N>(G, tw, nphi, nphi, partials, s, sc);
+  const int nrowblk = x_sched<T, N>(F, tw, norm, nphi, partials, s, sc);
   if (ev) (void)hipEventRecord(ev[3], s);
   hipLaunchKernelGGL(k_sigma_finalize, dim3(1), dim3(1024), 0, s, partials, nrowblk,
                      (double)N * N * N, sig);
@@ -386,21 +426,22 @@ static hipError_t launch_fields_n(cx<T> *F, const FillParams &f, const cx<T> *tw
 }
 
 template <typename T>
-hipError_t launch_fields(int n, cx<T> *F, const FillParams &f, const cx<T> *tw, double norm,
+hipError_t launch_fields(int n, cx<T> *F, cx<T> *G, const FillParams &f, const cx<T> *tw, double norm,
                          double *partials, double *sig, hipStream_t s, hipEvent_t *ev, const Schedule &sc)
 {
+  if (f.vmode != 0 && (!G || !f.sin_dx)) return hipErrorInvalidValue;
   switch (n) {
 #define CRIME_CASE(NN) \
-  case NN: return launch_fields_n<T, NN>(F, f, tw, norm, partials, sig, s, ev, sc);
+  case NN: return launch_fields_n<T, NN>(F, G, f, tw, norm, partials, sig, s, ev, sc);
     CRIME_FOR_EACH_N(CRIME_CASE)
 #undef CRIME_CASE
   default: return hipErrorInvalidValue;
   }
 }
 
-template hipError_t launch_fields<float>(int, cx<float> *, const FillParams &, const cx<float> *, double,
-                                         double *, double *, hipStream_t, hipEvent_t *, const Schedule &);
-template hipError_t launch_fields<double>(int, cx<double> *, const FillParams &, const cx<double> *,
+template hipError_t launch_fields<float>(int, cx<float> *, cx<float> *, const FillParams &, const cx<float> *,
+                                         double, double *, double *, hipStream_t, hipEvent_t *, const Schedule &);
+template hipError_t launch_fields<double>(int, cx<double> *, cx<double> *, const FillParams &, const cx<double> *,
                                           double, double *, double *, hipStream_t, hipEvent_t *, const Schedule &);
 
 template <typename T>

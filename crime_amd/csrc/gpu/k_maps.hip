@@ -42,7 +42,8 @@ __device__ __forceinline__ void cell_coords(const MapParams &m, size_t cell, int
   iz = (int)(q / n);
 }
 
-// centre, r and v_r (fourier.c:327-368); phi lives in the imaginary part of F
+// centre, r and v_r (fourier.c:327-368).  STENCIL: phi lives in the imaginary part of F and the
+// velocity is its central difference; SPECTRAL: vx is the imaginary part of F and (vy, vz) are G.
 template <typename T>
 __device__ __forceinline__ void cell_velocity(const MapParams &m, const cx<T> *__restrict__ F, int ix,
                                               int iy, int iz, CellState &s)
@@ -51,18 +52,27 @@ __device__ __forceinline__ void cell_velocity(const MapParams &m, const cx<T> *_
   s.x0 = m.dx * (ix + 0.5) - m.obs[0];
   s.y0 = m.dx * (iy + 0.5) - m.obs[1];
   s.z0 = m.dx * (iz + 0.5) - m.obs[2];
-  int ixp = ix == n - 1 ? 0 : ix + 1, ixm = ix == 0 ? n - 1 : ix - 1;
-  int iyp = iy == n - 1 ? 0 : iy + 1, iym = iy == 0 ? n - 1 : iy - 1;
-  int izp = iz == n - 1 ? 0 : iz + 1, izm = iz == 0 ? n - 1 : iz - 1;
   const size_t nn = (size_t)n, n2 = nn * nn;
   const size_t row = nn * iy + n2 * iz;
-  double px1 = (double)F[row + ixp].y, px0 = (double)F[row + ixm].y;
-  double py1 = (double)F[ix + nn * iyp + n2 * iz].y, py0 = (double)F[ix + nn * iym + n2 * iz].y;
-  double pz1 = (double)F[ix + nn * iy + n2 * izp].y, pz0 = (double)F[ix + nn * iy + n2 * izm].y;
-  double idx = 1. / m.dx;
-  double v0 = 0.5 * idx * (px1 - px0);
-  double v1 = 0.5 * idx * (py1 - py0);
-  double v2 = 0.5 * idx * (pz1 - pz0);
+  double v0, v1, v2;
+  if (m.vmode) {
+    const cx<T> g = ((const cx<T> *)m.G)[row + ix];
+    v0 = (double)F[row + ix].y;
+    v1 = (double)g.x;
+    v2 = (double)g.y;
+  }
+  else {
+    int ixp = ix == n - 1 ? 0 : ix + 1, ixm = ix == 0 ? n - 1 : ix - 1;
+    int iyp = iy == n - 1 ? 0 : iy + 1, iym = iy == 0 ? n - 1 : iy - 1;
+    int izp = iz == n - 1 ? 0 : iz + 1, izm = iz == 0 ? n - 1 : iz - 1;
+    double px1 = (double)F[row + ixp].y, px0 = (double)F[row + ixm].y;
+    double py1 = (double)F[ix + nn * iyp + n2 * iz].y, py0 = (double)F[ix + nn * iym + n2 * iz].y;
+    double pz1 = (double)F[ix + nn * iy + n2 * izp].y, pz0 = (double)F[ix + nn * iy + n2 * izm].y;
+    double idx = 1. / m.dx;
+    v0 = 0.5 * idx * (px1 - px0);
+    v1 = 0.5 * idx * (py1 - py0);
+    v2 = 0.5 * idx * (pz1 - pz0);
+  }
   double rr2 = s.x0 * s.x0 + s.y0 * s.y0 + s.z0 * s.z0;
   double irr = 1. / sqrt(rr2);
   s.r = sqrt(rr2);
@@ -153,6 +163,13 @@ __device__ __forceinline__ bool cell_may_bin(const MapParams &m, const CellState
 
 // Add val to bin `key` of the workgroup's LDS table (open addressing, linear probing); a sample that
 // finds no slot within the probe budget goes straight to the map in HBM.
+template <typename T>
+__device__ __forceinline__ void global_add(T *p, T v)
+{
+  if constexpr (std::is_same<T, unsigned long long>::value) atomicAdd(p, v);
+  else unsafeAtomicAdd(p, v);
+}
+
 template <typename T, int SLOTS>
 __device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned long long *keys, T *vals,
                                         T *__restrict__ maps)
@@ -176,19 +193,25 @@ __device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned 
     }
     h = (h + 1) & (SLOTS - 1);
   }
-  unsafeAtomicAdd(maps + key, val);
+  global_add(maps + key, val);
 }
 
 // One workgroup = one TILE_X x TILE_Y x TILE_Z block of cells (one cell per thread).  Samples are
 // summed per (channel, pixel) bin in LDS and every occupied bin is added to HBM once, so the
 // global atomics scale with the bins a tile touches instead of with its samples.  SLOTS (the
 // schedule variant: 2048, 1024 or 4096 bins) trades occupancy against probe length and overflow.
-template <typename T, int SLOTS>
+// FX (deterministic scatter): every sample adds the integer round(s mass / n_sub) to the 64-bit bin of
+// m.fxmaps instead of its FP mK value to maps; k_fx_finalize applies 1/s and the channel factor.
+template <typename T, int SLOTS, bool FX = false>
 __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T *__restrict__ maps,
                                                   MapParams m)
 {
+  using V = typename std::conditional<FX, unsigned long long, T>::type;
   __shared__ unsigned long long keys[SLOTS];
-  __shared__ T vals[SLOTS];
+  __shared__ V vals[SLOTS];
+  V *__restrict__ vmaps;
+  if constexpr (FX) vmaps = m.fxmaps;
+  else vmaps = maps;
   const int n = m.n;
   const int t = threadIdx.x;
   // tiles run in order of distance from the observer, so the workgroups resident at one time cover a
@@ -210,18 +233,24 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
   if (!__syncthreads_or(active)) return;  // no cell of the tile reaches a channel
   for (int i = t; i < SLOTS; i += 256) {
     keys[i] = ~0ull;
-    vals[i] = (T)0;
+    vals[i] = (V)0;
   }
   __syncthreads();
   if (active) {
     size_t cell = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
     cell_hi(m, (double)F[cell].x, m.sig[0], s);
     double mass_sub = s.mass / m.n_sub;
+    V qsub = (V)0;
+    if constexpr (FX) qsub = __double2ull_rn(mass_sub * m.fxs[0]);
     // npix = 12 Nside^2 < 2^32: the key is one 32 x 32 -> 64-bit multiply-add
     auto add = [&](int inu, int64_t ipix, double pf) {
-      if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix)
-        agg_add<T, SLOTS>((unsigned long long)(unsigned)inu * (unsigned)m.npix + (unsigned)ipix, (T)(mass_sub * pf),
-                          keys, vals, maps);
+      if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix) {
+        V v;
+        if constexpr (FX) v = qsub;
+        else v = (T)(mass_sub * pf);
+        agg_add<V, SLOTS>((unsigned long long)(unsigned)inu * (unsigned)m.npix + (unsigned)ipix, v, keys, vals,
+                          vmaps);
+      }
     };
     if (m.nu_mode == 0) {
       // two samples per iteration: their loads and FP64 chains overlap (62.4 against 64.7 ms at 1024^3)
@@ -256,8 +285,70 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
   __syncthreads();
   for (int i = t; i < SLOTS; i += 256) {
     unsigned long long k = keys[i];
-    if (k != ~0ull) unsafeAtomicAdd(maps + k, vals[i]);
+    if (k != ~0ull) global_add(vmaps + k, vals[i]);
   }
+}
+
+// Deterministic scatter, step 1: the box's HI mass (the same cell_hi as the map stage) summed in a fixed
+// order -- a fixed grid of FX_PARTS blocks, each thread over a fixed stride of cells, then a tree in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mass_partials(const cx<T> *__restrict__ F, MapParams m)
+{
+  __shared__ double red[256];
+  const size_t ncell = (size_t)m.n * m.n * m.n;
+  double acc = 0;
+  for (size_t cell = (size_t)blockIdx.x * 256 + threadIdx.x; cell < ncell; cell += (size_t)FX_PARTS * 256) {
+    int ix, iy, iz;
+    cell_coords<T>(m, cell, ix, iy, iz);
+    CellState s;
+    s.x0 = m.dx * (ix + 0.5) - m.obs[0];
+    s.y0 = m.dx * (iy + 0.5) - m.obs[1];
+    s.z0 = m.dx * (iz + 0.5) - m.obs[2];
+    s.r = sqrt(s.x0 * s.x0 + s.y0 * s.y0 + s.z0 * s.z0);
+    s.vr = 0;
+    cell_hi(m, (double)F[cell].x, m.sig[0], s);
+    acc += s.mass;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) m.mass_part[blockIdx.x] = red[0];
+}
+
+// step 2: total -> the power of two s with s total in (2^61, 2^62]; fxs = {s, 1/s}
+__global__ __launch_bounds__(256) void k_fx_scale(MapParams m)
+{
+  __shared__ double red[256];
+  double acc = 0;
+  for (int i = threadIdx.x; i < FX_PARTS; i += 256) acc += m.mass_part[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double tot = red[0];
+    int e = 0;
+    (void)frexp(tot > 0 ? tot : 1.0, &e);  // tot in [2^(e-1), 2^e)
+    m.fxs[0] = ldexp(1.0, 62 - e);          // s tot in [2^61, 2^62)
+    m.fxs[1] = ldexp(1.0, e - 62);
+  }
+}
+
+// step 4: maps[c][p] = (fx[c][p] / s) * prefac[c]
+template <typename T>
+__global__ __launch_bounds__(256) void k_fx_finalize(const unsigned long long *__restrict__ fx, T *__restrict__ maps,
+                                                     MapParams m)
+{
+  const int c = blockIdx.y;
+  const double f = m.prefac[c], inv = m.fxs[1];
+  const size_t base = (size_t)c * m.npix;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < m.npix; p += (int64_t)gridDim.x * 256)
+    maps[base + p] = (T)((double)fx[base + p] * inv * f);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -500,16 +591,17 @@ __global__ __launch_bounds__(256) void k_zero(T *__restrict__ p, size_t n)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_cell_field(const cx<T> *__restrict__ F, MapParams m, size_t ncell,
-                                                    int which, double *__restrict__ out)
+__global__ __launch_bounds__(256) void k_cell_field(const cx<T> *__restrict__ F, MapParams m, size_t cell0,
+                                                    size_t ncell, int which, double *__restrict__ out)
 {
-  for (size_t cell = (size_t)blockIdx.x * 256 + threadIdx.x; cell < ncell; cell += (size_t)gridDim.x * 256) {
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (size_t)gridDim.x * 256) {
+    const size_t cell = cell0 + c;
     int ix, iy, iz;
     cell_coords<T>(m, cell, ix, iy, iz);
     CellState s;
     cell_velocity<T>(m, F, ix, iy, iz, s);
     cell_hi(m, (double)F[cell].x, m.sig[0], s);
-    out[cell] = which == 2 ? s.vr : (which == 3 ? s.mass : s.dz);
+    out[c] = which == 2 ? s.vr : (which == 3 ? s.mass : s.dz);
   }
 }
 
@@ -532,6 +624,58 @@ __global__ __launch_bounds__(256) void k_sample_indices(const cx<T> *__restrict_
       int inu = sample_bin(m, s, isub, ipix, pf);
       inu_out[c * m.n_sub + isub] = inu;
       ipix_out[c * m.n_sub + isub] = ipix;
+    }
+  }
+}
+
+// Parity scatter: every sample of cells [cell0, cell0 + ncell) in FP64, through sample_bin.  ref != NULL:
+// whole maps with FP64 global atomics; otherwise per-channel sums in LDS (n_nu <= TOT_MAX_NU) flushed
+// to tot/cnt.
+constexpr int TOT_MAX_NU = 2048;
+template <typename T>
+__global__ __launch_bounds__(256) void k_parity_scatter(const cx<T> *__restrict__ F, MapParams m, int64_t cell0,
+                                                        int64_t ncell, double *__restrict__ ref,
+                                                        double *__restrict__ tot,
+                                                        unsigned long long *__restrict__ cnt)
+{
+  __shared__ double ht[TOT_MAX_NU];
+  __shared__ unsigned long long hc[TOT_MAX_NU];
+  if (!ref) {
+    for (int i = threadIdx.x; i < m.n_nu; i += 256) {
+      ht[i] = 0;
+      hc[i] = 0;
+    }
+    __syncthreads();
+  }
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * 256) {
+    size_t cell = (size_t)(cell0 + c);
+    int ix, iy, iz;
+    cell_coords<T>(m, cell, ix, iy, iz);
+    CellState s;
+    cell_velocity<T>(m, F, ix, iy, iz, s);
+    cell_hi(m, (double)F[cell].x, m.sig[0], s);
+    const double mass_sub = s.mass / m.n_sub;
+    for (int isub = 0; isub < m.n_sub; isub++) {
+      int64_t ipix;
+      double pf;
+      const int inu = sample_bin(m, s, isub, ipix, pf);
+      if (inu < 0 || (uint64_t)ipix >= (uint64_t)m.npix) continue;
+      if (ref) {
+        atomicAdd(ref + (size_t)inu * m.npix + ipix, mass_sub * pf);
+      }
+      else {
+        atomicAdd(&ht[inu], mass_sub * pf);
+        atomicAdd(&hc[inu], 1ull);
+      }
+    }
+  }
+  if (!ref) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < m.n_nu; i += 256) {
+      if (hc[i]) {
+        atomicAdd(tot + i, ht[i]);
+        atomicAdd(cnt + i, hc[i]);
+      }
     }
   }
 }
@@ -561,28 +705,72 @@ static inline unsigned nblk(size_t n)
   return (unsigned)(b < (1u << 20) ? b : (1u << 20));
 }
 
+template <typename T, bool FX>
+static void launch_pixelize(const cx<T> *F, T *maps, const MapParams &m, hipStream_t s, const Schedule &sc)
+{
+  size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
+  unsigned gx = (unsigned)(ntiles < 65536 ? ntiles : 65536);
+  unsigned gy = (unsigned)((ntiles + gx - 1) / gx);
+  if (sc.pix == 1) hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS / 2, FX>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
+  else if (sc.pix == 2) hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS * 2, FX>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
+  else hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS, FX>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
+}
+
+// ev[0] before clearing, ev[1] before the scatter, ev[2] after the last map kernel
 template <typename T>
 hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m, hipStream_t s,
                        hipEvent_t *ev, const Schedule &sc)
 {
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(nmaps)), dim3(256), 0, s, maps, nmaps);
+  if (m.fx) {
+    if (!m.fxmaps || !m.fxs || !m.mass_part) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(nblk(nmaps)), dim3(256), 0, s, m.fxmaps, nmaps);
+  }
+  else {
+    hipLaunchKernelGGL(k_zero<T>, dim3(nblk(nmaps)), dim3(256), 0, s, maps, nmaps);
+  }
   if (ev) (void)hipEventRecord(ev[1], s);
-  size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
-  unsigned gx = (unsigned)(ntiles < 65536 ? ntiles : 65536);
-  unsigned gy = (unsigned)((ntiles + gx - 1) / gx);
-  if (sc.pix == 1) hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS / 2>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
-  else if (sc.pix == 2) hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS * 2>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
-  else hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
+  if (m.fx) {
+    hipLaunchKernelGGL(k_mass_partials<T>, dim3(FX_PARTS), dim3(256), 0, s, F, m);
+    hipLaunchKernelGGL(k_fx_scale, dim3(1), dim3(256), 0, s, m);
+    launch_pixelize<T, true>(F, maps, m, s, sc);
+    int64_t b = (m.npix + 255) / 256;
+    hipLaunchKernelGGL(k_fx_finalize<T>, dim3((unsigned)(b < 4096 ? b : 4096), m.n_nu), dim3(256), 0, s,
+                       (const unsigned long long *)m.fxmaps, maps, m);
+  }
+  else {
+    launch_pixelize<T, false>(F, maps, m, s, sc);
+  }
   if (ev) (void)hipEventRecord(ev[2], s);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_cell_field(const cx<T> *F, const MapParams &m, int which, double *out, hipStream_t s)
+hipError_t launch_channel_totals(const cx<T> *F, const MapParams &m, int64_t cell0, int64_t ncell, double *tot,
+                                 unsigned long long *cnt, hipStream_t s)
 {
-  size_t ncell = (size_t)m.n * m.n * m.n;
-  hipLaunchKernelGGL(k_cell_field<T>, dim3(nblk(ncell)), dim3(256), 0, s, F, m, ncell, which, out);
+  if (m.n_nu > TOT_MAX_NU) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(256), 0, s, tot, (size_t)m.n_nu);
+  hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(1), dim3(256), 0, s, cnt, (size_t)m.n_nu);
+  hipLaunchKernelGGL(k_parity_scatter<T>, dim3(nblk((size_t)ncell)), dim3(256), 0, s, F, m, cell0, ncell,
+                     (double *)nullptr, tot, cnt);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_reference_maps(const cx<T> *F, const MapParams &m, double *ref, hipStream_t s)
+{
+  const int64_t ncell = (int64_t)m.n * m.n * m.n;
+  hipLaunchKernelGGL(k_parity_scatter<T>, dim3(nblk((size_t)ncell)), dim3(256), 0, s, F, m, (int64_t)0, ncell, ref,
+                     (double *)nullptr, (unsigned long long *)nullptr);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_cell_field(const cx<T> *F, const MapParams &m, int which, size_t cell0, size_t ncell, double *out,
+                             hipStream_t s)
+{
+  hipLaunchKernelGGL(k_cell_field<T>, dim3(nblk(ncell)), dim3(256), 0, s, F, m, cell0, ncell, which, out);
   return hipGetLastError();
 }
 
@@ -632,9 +820,13 @@ hipError_t launch_ps_expand(const double *A, const double *B, const double *c1, 
 #define CRIME_INST(T)                                                                                   \
   template hipError_t launch_maps<T>(const cx<T> *, T *, size_t, const MapParams &, hipStream_t,      \
                                      hipEvent_t *, const Schedule &);                                   \
-  template hipError_t launch_cell_field<T>(const cx<T> *, const MapParams &, int, double *, hipStream_t); \
+  template hipError_t launch_cell_field<T>(const cx<T> *, const MapParams &, int, size_t, size_t, double *,   \
+                                           hipStream_t);                                                \
   template hipError_t launch_sample_indices<T>(const cx<T> *, const MapParams &, int64_t, int64_t,     \
                                                int32_t *, int64_t *, hipStream_t);                      \
+  template hipError_t launch_channel_totals<T>(const cx<T> *, const MapParams &, int64_t, int64_t, double *, \
+                                               unsigned long long *, hipStream_t);                      \
+  template hipError_t launch_reference_maps<T>(const cx<T> *, const MapParams &, double *, hipStream_t);  \
   template hipError_t launch_grid_get<T>(const cx<T> *, size_t, int, double *, hipStream_t);          \
   template hipError_t launch_grid_set<T>(cx<T> *, size_t, int, const double *, hipStream_t);          \
   template hipError_t launch_psources<T>(const cx<T> *, const MapParams &, const PsParams &, hipStream_t); \

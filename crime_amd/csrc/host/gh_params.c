@@ -52,6 +52,8 @@ void gh_params_defaults(gh_params *p)
   p->n_subcell = 0;
   p->precision = CRIME_GH_FP32;
   p->rsd = 1;
+  p->velocity_mode = CRIME_GH_VEL_STENCIL;
+  p->deterministic = 0;
   p->mode_source = CRIME_GH_MODES_PHILOX;
   p->device = 0;
 }
@@ -120,6 +122,15 @@ int gh_params_read(gh_params *p, const char *fname)
       }
     }
     else if (!strcmp(s1, "rsd=")) p->rsd = atoi(s2) != 0;
+    else if (!strcmp(s1, "deterministic=")) p->deterministic = atoi(s2) != 0;
+    else if (!strcmp(s1, "velocity_mode=")) {
+      if (lower_eq(s2, "stencil") || !strcmp(s2, "0")) p->velocity_mode = CRIME_GH_VEL_STENCIL;
+      else if (lower_eq(s2, "spectral") || !strcmp(s2, "1")) p->velocity_mode = CRIME_GH_VEL_SPECTRAL;
+      else {
+        fclose(fi);
+        return set_err(p, "Unknown velocity_mode %s (stencil|spectral)", s2);
+      }
+    }
     else if (!strcmp(s1, "modes=")) {
       if (lower_eq(s2, "philox")) p->mode_source = CRIME_GH_MODES_PHILOX;
       else if (lower_eq(s2, "mt19937")) p->mode_source = CRIME_GH_MODES_INJECTED;
@@ -329,6 +340,8 @@ void gh_params_to_desc(const gh_params *p, crime_gh_desc *d)
   d->mode_source = p->mode_source;
   d->unit_modes = p->unit_modes;
   d->device = p->device;
+  d->velocity_mode = p->velocity_mode;
+  d->deterministic = p->deterministic;
 }
 
 double gh_fraction_hi(double z)
@@ -376,7 +389,8 @@ double gh_params_get(const gh_params *p, const char *name)
       {"l_box", p->l_box}, {"sigma8_in", p->sigma8_in}, {"fgrowth_0", p->fgrowth_0},
       {"hubble_0", p->hubble_0}, {"idr", p->idr}, {"nu_min", p->nu_min}, {"nu_max", p->nu_max},
       {"n_nu", p->n_nu}, {"n_grid", p->n_grid}, {"n_side", (double)p->n_side}, {"n_sub", p->n_sub},
-      {"precision", p->precision}, {"seed", p->seed}};
+      {"precision", p->precision}, {"seed", p->seed}, {"velocity_mode", p->velocity_mode},
+      {"deterministic", p->deterministic}};
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
     if (!strcmp(name, t[i].k)) return t[i].v;
   return NAN;

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CRIME_GH_ABI_VERSION 1u
+#define CRIME_GH_ABI_VERSION 2u   /* 2: velocity_mode appended to crime_gh_desc */
 #define CRIME_GH_NZ 5001            /* table length, NZ in src/common_gh.h:34 */
 #define CRIME_GH_DZ 0.001           /* z step of the z->r table, DZ in src/common_gh.h:33 */
 #define CRIME_GH_NU_21 1420.40575177 /* MHz, NU_21 in src/common.h:38 */
@@ -64,6 +64,15 @@ enum { CRIME_GH_NU_TABLE = 0, CRIME_GH_NU_UNIFORM = 1 };
 /* HI bias model.  REFERENCE = bias_HI(z)=0.904+0.135(1+z)^1.696 (src/user_defined.c:32-35);
  * CONST = b0; LINEAR = b0 + b1 (1+z).  The HI fraction is always fraction_HI (:27-30). */
 enum { CRIME_GH_BIAS_REFERENCE = 0, CRIME_GH_BIAS_CONST = 1, CRIME_GH_BIAS_LINEAR = 2 };
+
+/* Velocity field behind the RSD shift.
+ *  STENCIL (the reference): one inverse FFT of the velocity potential phi_k = delta_k f0 H0 / k^2
+ *    (src/fourier.c:297-298), then v_r = (central difference of phi) . r_hat (:307-373).
+ *  SPECTRAL: three inverse FFTs of the velocity components v_j(k) = i sin(k_j dx)/dx phi_k, the
+ *    Fourier symbol of that central difference (0 at Nyquist), projected on r_hat per cell.  The
+ *    two modes give the same v_r up to rounding; SPECTRAL holds vx, vy, vz as grids (one more
+ *    complex N^3 grid in HBM) and the map stage reads them instead of six neighbours of phi. */
+enum { CRIME_GH_VEL_STENCIL = 0, CRIME_GH_VEL_SPECTRAL = 1 };
 
 typedef struct crime_gh_desc {
   uint32_t abi_version;       /* CRIME_GH_ABI_VERSION */
@@ -104,6 +113,8 @@ typedef struct crime_gh_desc {
                                  unit-variance modes of the reference's fill order
                                  (src/fourier.c:260-302), used when mode_source is INJECTED */
   int32_t device;             /* HIP device ordinal */
+  int32_t velocity_mode;      /* CRIME_GH_VEL_* (ABI 2) */
+  int32_t deterministic;      /* 1: bit-reproducible maps (fixed-point scatter, see below) (ABI 2) */
 } crime_gh_desc;
 
 /* per-call timings (hipEvent, ms) and diagnostics */
@@ -128,7 +139,10 @@ enum {
   CRIME_GH_GRID_VPOT = 1,   /* normalised velocity potential (grid_vpot after :410) */
   CRIME_GH_GRID_RVEL = 2,   /* radial velocity (grid_rvel after fourier.c:368), read only */
   CRIME_GH_GRID_MASS = 3,   /* HI mass per cell (grid_dens after grid_tools.c:145), read only */
-  CRIME_GH_GRID_DZRSD = 4   /* RSD redshift shift (grid_rvel after grid_tools.c:146), read only */
+  CRIME_GH_GRID_DZRSD = 4,  /* RSD redshift shift (grid_rvel after grid_tools.c:146), read only */
+  CRIME_GH_GRID_VX = 5,     /* velocity components of CRIME_GH_VEL_SPECTRAL handles (vel[0..2] of */
+  CRIME_GH_GRID_VY = 6,     /* fourier.c:359-366 before the projection); VPOT is not held by them */
+  CRIME_GH_GRID_VZ = 7
 };
 
 /* point-source pass: timings (hipEvent, ms) and the number of sources drawn */
@@ -164,6 +178,17 @@ int crime_gh_mk_T_maps(crime_gh_t *h, crime_gh_stats *st);
 /* Both stages in order (src/main_gh.c:52-62). */
 int crime_gh_run(crime_gh_t *h, crime_gh_stats *st);
 
+/* Deterministic scatter (desc.deterministic = 1; no counterpart in the reference, whose FP32 OpenMP
+ * atomics make the maps depend on the thread schedule, src/pixelize.c:223-226).  The map stage first
+ * sums the HI mass of the whole box in a fixed order, takes the power of two s with
+ * s * (total mass) in [2^61, 2^62), and scatters every sample as the integer round(s * mass / n_sub)
+ * with 64-bit integer atomics, which are associative: the sums are the same whatever the order.  The
+ * maps are then (sum / s) * M2T/(r^2 dnu) per channel -- the reference's own order, which scales the
+ * finished mass maps (pixelize.c:246-261).  No bin can overflow: all bins together hold at most
+ * 2^62 + n_samples/2.  Quantisation: s^-1 <= 2^-61 of the total mass per sample.  The maps are
+ * bit-identical from run to run, on any launch order (batches included).  Costs one more read of
+ * delta, an 8-byte integer map buffer and a conversion pass. */
+
 /* Device-to-host copy of the maps as float (what write_maps writes, src/io_gh.c:69-82) or double. */
 int crime_gh_copy_maps(crime_gh_t *h, float *maps_out);
 int crime_gh_copy_maps_f64(crime_gh_t *h, double *maps_out);
@@ -172,8 +197,11 @@ int crime_gh_copy_maps_f64(crime_gh_t *h, double *maps_out);
 int crime_gh_get_sigma2(crime_gh_t *h, double *sigma2, double *mean);
 
 /* Parity accessors: read a grid (computing RVEL/MASS/DZRSD on demand from the resident fields),
- * or overwrite DENS/VPOT (and sigma^2) so the map stage can be fed the oracle's fields. */
+ * or overwrite DENS/VPOT (stencil mode) or DENS/VX/VY/VZ (spectral mode), and sigma^2, so the map
+ * stage can be fed the oracle's fields. */
 int crime_gh_copy_grid(crime_gh_t *h, int which, double *dst);
+/* the same for cells [cell0, cell0 + ncell) only (x-fastest order), for grids too large to copy whole */
+int crime_gh_copy_grid_range(crime_gh_t *h, int which, int64_t cell0, int64_t ncell, double *dst);
 int crime_gh_set_grid(crime_gh_t *h, int which, const double *src);
 int crime_gh_set_sigma2(crime_gh_t *h, double sigma2, double mean);
 
@@ -181,6 +209,16 @@ int crime_gh_set_sigma2(crime_gh_t *h, double sigma2, double mean);
  * each: inu[c*n_sub+s] (-1 when the sample falls outside every channel) and ipix[...] (-1 then). */
 int crime_gh_sample_indices(crime_gh_t *h, int64_t cell0, int64_t ncell, int32_t *inu,
                             int64_t *ipix);
+
+/* Parity accessors of the map stage at any size (no counterpart in the reference).  Both walk every
+ * sample of every cell through the same per-sample decisions as crime_gh_sample_indices -- no cell
+ * screen, no LDS aggregation -- and add mass/n_sub * M2T/(r^2 dnu) in FP64:
+ *  - crime_gh_channel_totals: per-channel sums over cells [cell0, cell0+ncell) into tot[n_nu], and
+ *    the number of binned samples per channel into count[n_nu] (may be NULL);
+ *  - crime_gh_reference_maps: the whole maps, FP64, out[n_nu * N_pix] (allocates N_pix n_nu doubles
+ *    on the device for the call). */
+int crime_gh_channel_totals(crime_gh_t *h, int64_t cell0, int64_t ncell, double *tot, int64_t *count);
+int crime_gh_reference_maps(crime_gh_t *h, double *out);
 
 /* Power spectrum of the resident Gaussian field delta (no counterpart in the reference): a forward
  * 3-D FFT of the grid and |delta_k|^2 L^3/N^6 averaged over shells |k| in [i, i+1) * bin_dk (h/Mpc),

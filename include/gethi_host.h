@@ -46,6 +46,8 @@ typedef struct gh_params {
   int n_subcell;            /* n_subcell= 0: 10 random shared offsets; k>0: k^3 regular sub-cells */
   int precision;            /* precision= single|double */
   int rsd;                  /* rsd= 1|0 */
+  int velocity_mode;        /* velocity_mode= stencil|spectral (CRIME_GH_VEL_*) */
+  int deterministic;        /* deterministic= 0|1 (bit-reproducible fixed-point scatter) */
   int mode_source;          /* modes= philox|mt19937  (mt19937: the reference's 1-thread stream,
                                injected into the device) */
   int device;               /* device= */

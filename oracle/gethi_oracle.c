@@ -9,6 +9,8 @@
  *   orc_c2r_3d            src/fourier.c:78-99   (FFTW c2r; own FP64 mixed-radix FFT)
  *   orc_fields            src/fourier.c:375-438 (normalisation :395-412, sigma^2 :24-76)
  *   orc_radial_velocity   src/fourier.c:307-373
+ *   orc_velocity_spectral the three components v_j(k) = i sin(k_j dx)/dx phi_k (the Fourier symbol
+ *                         of the central difference of src/fourier.c:359-366; no reference code)
  *   orc_get_hi            src/grid_tools.c:103-153, src/user_defined.c:27-35
  *   orc_mk_T_maps         src/pixelize.c:150-261 with get_inu :28-55 (the warm-started walk)
  *   orc_vec2pix_ring      chealpix vec2pix_ring (called at src/pixelize.c:222), restated from
@@ -341,6 +343,73 @@ void orc_radial_velocity(const crime_gh_desc *d, const double *vpot, double *rve
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* spectral velocity components (no counterpart in the reference): v_j(k) = i sin(k_j dx)/dx phi_k, */
+/* the Fourier symbol of the central difference of src/fourier.c:359-366, zero at Nyquist          */
+/* ------------------------------------------------------------------------------------------ */
+static double sin_dx(int m, int n, double dx)
+{
+  /* sin(2 pi m / n) / dx with exact zeros at m = 0 and n/2, odd under m -> n - m */
+  m %= n;
+  if (m == 0 || 2 * m == n) return 0;
+  if (2 * m > n) return -sin_dx(n - m, n, dx);
+  return (double)(sinl(2.0L * 3.14159265358979323846264338327950288L * m / n) / (long double)dx);
+}
+
+int orc_velocity_spectral(const crime_gh_desc *d, double *vx, double *vy, double *vz)
+{
+  const int n = d->n_grid, nh = n / 2 + 1;
+  const size_t nhalf = (size_t)n * n * nh, n3 = (size_t)n * n * n;
+  const double dx = d->l_box / n, norm = pow(sqrt(2 * M_PI) / d->l_box, 3);
+  double *dk = (double *)malloc(2 * nhalf * sizeof(double));
+  double *pk = (double *)malloc(2 * nhalf * sizeof(double));
+  double *vk = (double *)malloc(2 * nhalf * sizeof(double));
+  int rc = (!dk || !pk || !vk) ? -2 : 0;
+  if (!rc) orc_fill_modes(d, dk, pk);
+  double *out[3] = {vx, vy, vz};
+  for (int j = 0; j < 3 && !rc; j++) {
+#pragma omp parallel for schedule(static)
+    for (int ii = 0; ii < n; ii++)
+      for (int jj = 0; jj < n; jj++)
+        for (int kk = 0; kk < nh; kk++) {
+          size_t idx = (size_t)kk + (size_t)nh * ((size_t)jj + (size_t)n * ii);
+          double s = sin_dx(j == 0 ? kk : (j == 1 ? jj : ii), n, dx);
+          /* i s phi_k */
+          vk[2 * idx] = -s * pk[2 * idx + 1];
+          vk[2 * idx + 1] = s * pk[2 * idx];
+        }
+    rc = orc_c2r_3d(n, vk, out[j]);
+    if (!rc)
+      for (size_t i = 0; i < n3; i++) out[j][i] *= norm;
+  }
+  free(dk);
+  free(pk);
+  free(vk);
+  return rc;
+}
+
+void orc_radial_velocity_spectral(const crime_gh_desc *d, const double *vx, const double *vy, const double *vz,
+                                  double *rvel)
+{
+  /* the projection of src/fourier.c:327-368 with the velocity components given */
+  const int n = d->n_grid;
+  const size_t nn = n, n2 = nn * nn;
+  const double dx = d->l_box / n;
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < n; iz++) {
+    double z = dx * (iz + 0.5) - d->pos_obs[2];
+    for (int iy = 0; iy < n; iy++) {
+      double y = dx * (iy + 0.5) - d->pos_obs[1];
+      for (int ix = 0; ix < n; ix++) {
+        size_t i = ix + nn * iy + n2 * iz;
+        double x = dx * (ix + 0.5) - d->pos_obs[0];
+        double irr = 1. / sqrt(x * x + y * y + z * z);
+        rvel[i] = vx[i] * (x * irr) + vy[i] * (y * irr) + vz[i] * (z * irr);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* get_HI: src/grid_tools.c:103-153                                                            */
 /* ------------------------------------------------------------------------------------------ */
 double orc_bias(const crime_gh_desc *d, double z)
@@ -550,8 +619,14 @@ int orc_run(const crime_gh_desc *d, double *maps, double *sigma2, double *mean)
   double *dz = (double *)malloc(n3 * sizeof(double));
   int rc = (!dens || !vpot || !rvel || !mass || !dz) ? -2 : 0;
   if (!rc) rc = orc_fields(d, dens, vpot, sigma2, mean);
-  if (!rc) {
+  if (!rc && d->velocity_mode == CRIME_GH_VEL_SPECTRAL) {
+    /* vpot, mass and dz serve as the three components before they are needed */
+    rc = orc_velocity_spectral(d, vpot, mass, dz);
+    if (!rc) orc_radial_velocity_spectral(d, vpot, mass, dz, rvel);
+  }
+  else if (!rc)
     orc_radial_velocity(d, vpot, rvel);
+  if (!rc) {
     orc_get_hi(d, dens, rvel, *sigma2, mass, dz);
     rc = orc_mk_T_maps(d, mass, dz, maps);
   }

@@ -18,6 +18,9 @@ int orc_c2r_3d(int n, const double *half, double *out);
 void orc_sigma2(int n, const double *dens, double *sigma2, double *mean);
 int orc_fields(const crime_gh_desc *d, double *dens, double *vpot, double *sigma2, double *mean);
 void orc_radial_velocity(const crime_gh_desc *d, const double *vpot, double *rvel);
+int orc_velocity_spectral(const crime_gh_desc *d, double *vx, double *vy, double *vz);
+void orc_radial_velocity_spectral(const crime_gh_desc *d, const double *vx, const double *vy, const double *vz,
+                                  double *rvel);
 double orc_bias(const crime_gh_desc *d, double z);
 void orc_get_hi(const crime_gh_desc *d, const double *dens, const double *rvel, double sigma2, double *mass,
                 double *dzrsd);

@@ -35,6 +35,9 @@ def lib():
         L.orc_fields.argtypes = [D, _dp, _dp, _dp, _dp]
         L.orc_fields.restype = C.c_int
         L.orc_radial_velocity.argtypes = [D, _dp, _dp]
+        L.orc_velocity_spectral.argtypes = [D, _dp, _dp, _dp]
+        L.orc_velocity_spectral.restype = C.c_int
+        L.orc_radial_velocity_spectral.argtypes = [D, _dp, _dp, _dp, _dp]
         L.orc_get_hi.argtypes = [D, _dp, _dp, C.c_double, _dp, _dp]
         L.orc_vec2pix_ring.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double]
         L.orc_vec2pix_ring.restype = C.c_int64
@@ -136,6 +139,22 @@ def radial_velocity(desc, vpot):
     return out
 
 
+def velocity_spectral(desc):
+    """(vx, vy, vz) from v_j(k) = i sin(k_j dx)/dx phi_k of the descriptor's modes."""
+    n = desc.n_grid
+    v = [np.empty((n, n, n)) for _ in range(3)]
+    rc = lib().orc_velocity_spectral(C.byref(desc), *[_p(a) for a in v])
+    assert rc == 0
+    return v
+
+
+def radial_velocity_spectral(desc, vx, vy, vz):
+    out = np.empty_like(vx, dtype=np.float64)
+    lib().orc_radial_velocity_spectral(C.byref(desc), *[_p(np.ascontiguousarray(a, np.float64)) for a in (vx, vy, vz)],
+                                       _p(out))
+    return out
+
+
 def get_hi(desc, dens, rvel, sigma2):
     mass = np.empty(dens.shape)
     dz = np.empty(dens.shape)
@@ -163,6 +182,19 @@ def sample_indices(desc, dzrsd, cell0, ncell):
     ipix = np.empty((ncell, desc.n_sub), np.int64)
     lib().orc_sample_indices(C.byref(desc), _p(np.ascontiguousarray(dzrsd, np.float64)), cell0, ncell,
                              inu.ctypes.data_as(C.POINTER(C.c_int32)), ipix.ctypes.data_as(C.POINTER(C.c_int64)))
+    return inu, ipix
+
+
+def sample_indices_slab(desc, dz_slab, cell0, ncell):
+    """sample_indices with the RSD shifts of cells [cell0, cell0 + ncell) only (dz_slab[c] is cell
+    cell0 + c): the C code reads dzrsd[cell0 .. cell0 + ncell) through a pointer moved back by cell0."""
+    dz = np.ascontiguousarray(dz_slab, np.float64)
+    assert dz.size == ncell
+    inu = np.empty((ncell, desc.n_sub), np.int32)
+    ipix = np.empty((ncell, desc.n_sub), np.int64)
+    base = C.cast(dz.ctypes.data - 8 * int(cell0), _dp)
+    lib().orc_sample_indices(C.byref(desc), base, cell0, ncell, inu.ctypes.data_as(C.POINTER(C.c_int32)),
+                             ipix.ctypes.data_as(C.POINTER(C.c_int64)))
     return inu, ipix
 
 

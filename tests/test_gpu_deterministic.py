@@ -1,0 +1,66 @@
+"""Deterministic scatter (desc.deterministic = 1): bit-identical maps from run to run, in a batch graph
+as in a single run, and the same maps as the floating-point scatter up to its quantisation.
+
+The reference's maps depend on the order of its OpenMP FP32 atomics (src/pixelize.c:223-226); the
+fixed-point scatter sums integers round(s mass / n_sub), whose atomic adds commute.
+"""
+import numpy as np
+import pytest
+
+import crime_amd
+import oracle
+from conftest import small_params
+from test_gpu_parity import rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_deterministic_bit_identical_runs(cfgdir, prec):
+    print(f"deterministic scatter, {prec}: repeated runs", flush=True)
+    par = small_params(cfgdir, f"det_{prec}", n_grid=64, n_side=32, precision=prec,
+                       extra=dict(deterministic=1))
+    assert par.get("deterministic") == 1
+    with crime_amd.GetHI(par) as gh:
+        gh.run()
+        a = gh.maps(np.float64)
+        for v in (1, 2, 0):  # other LDS table sizes change the order of every atomic
+            gh.set_schedule([0, 0, 0, v])
+            gh.run()
+            b = gh.maps(np.float64)
+            assert np.array_equal(a, b), v
+    assert (a != 0).sum() > 10000
+
+
+def test_deterministic_vs_float_scatter_and_oracle(cfgdir):
+    par = small_params(cfgdir, "det_vs", n_grid=48, n_side=16)
+    om, _, _ = oracle.run(par.desc())
+    with crime_amd.GetHI(par, deterministic=1) as gh:
+        gh.run()
+        m = gh.maps(np.float64)
+    assert np.array_equal(m != 0, om != 0)
+    err = rel(m, om)
+    print(f"deterministic FP64 maps vs oracle: max rel {err:.2e}")
+    assert err < 1e-10
+
+
+def test_deterministic_batch_equals_single(cfgdir):
+    pars = [small_params(cfgdir, f"det_b{i}", n_grid=n, n_side=16, seed=50 + i, precision="single",
+                         extra=dict(deterministic=1))
+            for i, n in enumerate((32, 48, 64))]
+    hs = [crime_amd.GetHI(p) for p in pars]
+    try:
+        single = []
+        for h in hs:
+            h.run()
+            single.append(h.maps())
+        with crime_amd.Batch(hs, n_streams=3) as b:
+            for rep in range(3):
+                for h in hs:  # eager work between replays
+                    h.create_d_and_vr_fields()
+                b.run()
+                for h, ref in zip(hs, single):
+                    assert np.array_equal(h.maps(), ref), rep
+    finally:
+        for h in hs:
+            h.close()

@@ -107,3 +107,61 @@ def test_two_modes_superpose(cfgdir):
     assert np.abs(dens - (a + b)).max() < 1e-12 * s
     ref_v = (a / ka + b / kb) * d.f0h0
     assert np.abs(vpot - ref_v).max() < 1e-12 * np.abs(ref_v).max()
+
+
+# ---------------------------------------------------------------------------------------------
+# spectral velocity components: v_j(k) = i sin(k_j dx)/dx phi_k (the symbol of fourier.c:359-366)
+# ---------------------------------------------------------------------------------------------
+def _central_difference(vpot, dx):
+    """(phi(x + dx e_j) - phi(x - dx e_j)) / (2 dx) per axis, periodic; arrays are [z, y, x]."""
+    return [(np.roll(vpot, -1, axis=a) - np.roll(vpot, 1, axis=a)) / (2 * dx) for a in (2, 1, 0)]
+
+
+@pytest.mark.parametrize("mode", [(3, 5, 2), (0, 2, 3), (N // 2, 1, 4), (5, N // 2, N - 3), (1, 7, N // 2)])
+def test_spectral_velocity_single_mode(cfgdir, mode):
+    par = small_params(cfgdir, "onemode_v", n_grid=N, l_box=600.0)
+    g = 0.6 - 1.3j
+    d, dens, vpot = _single_mode_fields(par, *mode, g)
+    vel = oracle.velocity_spectral(d)
+    delta, k2 = _expected(d, *mode, g)
+    dx = d.l_box / N
+    i = np.arange(N)
+    Z, Y, X = np.meshgrid(i, i, i, indexing="ij")
+    phase = 2 * math.pi * (mode[0] * X + mode[1] * Y + mode[2] * Z) / N
+    # delta = C Re(g e^{i phase}) with C = norm c A: v_j = C f0H0/k^2 Re(i sin(2 pi m_j/N)/dx g e^{i phase})
+    C = np.abs(delta).max() / np.abs(np.real(g * np.exp(1j * phase))).max()
+    scale = C * d.f0h0 / k2 / dx
+    for j, m in enumerate(mode):
+        ref = C * d.f0h0 / k2 * np.real(1j * math.sin(2 * math.pi * m / N) / dx * g * np.exp(1j * phase))
+        assert np.abs(vel[j] - ref).max() < 1e-12 * scale, j
+        if 2 * m == N:
+            assert np.abs(vel[j]).max() < 1e-12 * scale  # zero at Nyquist, like the stencil
+    for j, cd in enumerate(_central_difference(vpot, dx)):
+        assert np.abs(vel[j] - cd).max() < 1e-12 * scale, j
+
+
+@pytest.mark.parametrize("n", [16, 24])
+def test_spectral_radial_velocity_equals_stencil(cfgdir, n):
+    """On a full random field the spectral v_r equals the reference's stencil v_r to rounding."""
+    par = small_params(cfgdir, f"vspec_{n}", n_grid=n, r_smooth=5.0)
+    d = par.desc()
+    dens, vpot, _, _ = oracle.fields(d)
+    vel = oracle.velocity_spectral(d)
+    dx = d.l_box / n
+    for j, cd in enumerate(_central_difference(vpot, dx)):
+        assert np.abs(vel[j] - cd).max() < 1e-12 * np.abs(cd).max(), j
+    rv_spec = oracle.radial_velocity_spectral(d, *vel)
+    rv_sten = oracle.radial_velocity(d, vpot)
+    assert np.abs(rv_spec - rv_sten).max() < 1e-12 * np.abs(rv_sten).max()
+
+
+def test_run_spectral_equals_stencil_maps(cfgdir):
+    """The whole oracle pipeline gives the same maps in both velocity modes, up to samples whose
+    channel flips with the last-bit difference of v_r (none expected at this size)."""
+    par = small_params(cfgdir, "vspec_run", n_grid=16, n_side=8, n_nu=8)
+    d = par.desc()
+    a, s2a, _ = oracle.run(d)
+    d.velocity_mode = 1
+    b, s2b, _ = oracle.run(d)
+    assert abs(s2a - s2b) < 1e-13 * s2a  # OpenMP reduction order
+    assert np.abs(a - b).max() < 1e-10 * np.abs(a).max()
