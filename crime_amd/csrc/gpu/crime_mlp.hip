@@ -28,7 +28,7 @@ constexpr int KSPLIT = 256;                           // batch rows per split of
 constexpr int RROWS = 32;                             // batch rows per partial of the column reductions
 constexpr uint32_t MLP_KEY1 = 0x4D4C5031u;            // second Philox key word of the minibatch draws
 
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // per-handle device state read by the captured step
@@ -53,7 +53,7 @@ __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + expf(-z))
 
 // ---------------------------------------------------------------------------------------------
 // BF16 GEMM C[M x N] = A[M x K] B[K x N] with FP32 accumulation; TA: A is stored K x M, TB: B is
-// stored N x K.  64 x 64 output tile per workgroup (4 waves of 2 x 2 v_mfma_f32_16x16x16bf16_1k),
+// stored N x K.  64 x 64 output tile per workgroup (4 waves of 2 x 2 v_mfma_f32_16x16x32_bf16),
 // K in chunks of 32 through LDS, split-K over blockIdx.z (rows [z kchunk, (z+1) kchunk) of K).
 // MFMA operands: lane = i + 16 (k / 4) holds A[i][k] (element k % 4), lane = j + 16 (k / 4) holds
 // B[k][j]; D[i][j] is element i % 4 of lane j + 16 (i / 4).  N and K are multiples of 64 and 32.
@@ -97,13 +97,16 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, in
       *(uint4 *)(Bs + n * LP + k8) = *(const uint4 *)(B + (size_t)(n0 + n) * ldb + k0 + k8);
     }
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < TK; kk += 16) {
-      s4 af[2], bfr[2];
-      for (int i = 0; i < 2; i++) af[i] = *(const s4 *)(As + (wr + 16 * i + (lane & 15)) * LP + kk + 4 * (lane >> 4));
-      for (int j = 0; j < 2; j++) bfr[j] = *(const s4 *)(Bs + (wc + 16 * j + (lane & 15)) * LP + kk + 4 * (lane >> 4));
+    // gfx950's v_mfma_f32_16x16x32_bf16: the whole 32-deep k tile in one instruction per 16 x 16 block.
+    // Lane l supplies A[row l % 16][k 8 (l / 16) .. +7] and B[k 8 (l / 16) .. +7][column l % 16].
+    {
+      bf16x8 af[2], bfr[2];
       for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        af[i] = __builtin_bit_cast(bf16x8, *(const uint4 *)(As + (wr + 16 * i + (lane & 15)) * LP + 8 * (lane >> 4)));
+      for (int j = 0; j < 2; j++)
+        bfr[j] = __builtin_bit_cast(bf16x8, *(const uint4 *)(Bs + (wc + 16 * j + (lane & 15)) * LP + 8 * (lane >> 4)));
+      for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
@@ -320,6 +323,7 @@ struct crime_mlp {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_batch = 0;
+  int use_graph = 1;                        // 0: every step enqueued directly (crime_mlp_set_graph)
   std::string err;
 };
 
@@ -627,6 +631,13 @@ extern "C" int crime_mlp_set_data(crime_mlp_t *h, int64_t n, const float *x, con
   return 0;
 }
 
+extern "C" int crime_mlp_set_graph(crime_mlp_t *h, int on)
+{
+  if (!h) return mlp_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  h->use_graph = on ? 1 : 0;
+  return 0;
+}
+
 extern "C" int crime_mlp_train(crime_mlp_t *h, int steps, int batch, float lr0, int total_steps, float beta1,
                                float beta2, float eps, uint32_t seed, float *loss_out)
 {
@@ -651,6 +662,19 @@ extern "C" int crime_mlp_train(crime_mlp_t *h, int steps, int batch, float lr0, 
   s.ndata = h->ndata;
   MLP_HIP(h, hipMemcpy(h->st, &s, sizeof(s), hipMemcpyHostToDevice));
   MLP_HIP(h, hipMemset(h->loss, 0, (size_t)steps * sizeof(float)));
+  if (!h->use_graph) {
+    for (int i = 0; i < steps; i++) {
+      hipLaunchKernelGGL(k_gather, dim3(nblk(batch)), dim3(256), 0, h->stream, h->Xall, h->Tall, h->Wall, batch, h->st,
+                         h->X, h->T, h->Wt);
+      enqueue_forward(h, batch, 1);
+      enqueue_backward(h, batch);
+      enqueue_adam(h, batch);
+      MLP_HIP(h, hipGetLastError());
+    }
+    MLP_HIP(h, hipStreamSynchronize(h->stream));
+    if (loss_out) MLP_HIP(h, hipMemcpy(loss_out, h->loss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+  }
   if (h->exec && h->graph_batch != batch) drop_graph(h);
   if (!h->exec) {
     MLP_HIP(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));

@@ -100,7 +100,10 @@ extern "C" void crime_sht_destroy(crime_sht_t *p)
 {
   if (!p) return;
   (void)hipSetDevice(p->device);
+  // every stream that may still run ring kernels reading these buffers is drained before the frees
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (int k = 0; k < 2; k++)
+    if (p->side[k]) (void)hipStreamSynchronize(p->side[k]);
   void *bufs[] = {p->d_rings, p->d_z, p->d_st, p->d_coef, p->d_off, p->d_tiles, p->d_syn_tiles, p->d_sig2,
                   p->d_mask, (void *)p->d_mtab, p->d_G,
                   p->d_alm, p->d_lam, p->d_cls, p->d_unpack, p->d_hmaps, p->d_tw,
@@ -109,7 +112,6 @@ extern "C" void crime_sht_destroy(crime_sht_t *p)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   for (int k = 0; k < 2; k++) {
-    if (p->side[k]) (void)hipStreamSynchronize(p->side[k]);
     if (p->join[k]) (void)hipEventDestroy(p->join[k]);
     if (p->side[k]) (void)hipStreamDestroy(p->side[k]);
   }
@@ -429,16 +431,22 @@ static int run_ring_groups(crime_sht *p, Launch launch)
 {
   SHT_HIP(p, hipEventRecord(p->fork, p->stream));
   for (int k = 0; k < 2; k++) SHT_HIP(p, hipStreamWaitEvent(p->side[k], p->fork, 0));
-  for (size_t i = 0; i < p->groups.size(); i++) {
+  int rc = 0;
+  for (size_t i = 0; i < p->groups.size() && !rc; i++) {
     const hipStream_t s = i % 3 == 0 ? p->stream : p->side[i % 3 - 1];
-    const int rc = launch(p->groups[i], s);
-    if (rc) return rc;
+    rc = launch(p->groups[i], s);
   }
+  // join the side streams into the plan's stream also after a failed launch, so that work already
+  // queued on them is ordered before anything the caller does next (a destroy included)
   for (int k = 0; k < 2; k++) {
-    SHT_HIP(p, hipEventRecord(p->join[k], p->side[k]));
-    SHT_HIP(p, hipStreamWaitEvent(p->stream, p->join[k], 0));
+    hipError_t e = hipEventRecord(p->join[k], p->side[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(p->stream, p->join[k], 0);
+    if (e != hipSuccess && !rc) {
+      (void)hipGetLastError();
+      rc = sht_error(p, CRIME_GH_EHIP, "ring groups join: %s", hipGetErrorString(e));
+    }
   }
-  return 0;
+  return rc;
 }
 
 // ring spectra + alm GEMMs of nsets sets of mps device maps each (set s at d_maps[s]) into p->d_alm,

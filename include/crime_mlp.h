@@ -64,6 +64,10 @@ int crime_mlp_set_data(crime_mlp_t *h, int64_t n, const float *x, const float *t
 int crime_mlp_train(crime_mlp_t *h, int steps, int batch, float lr0, int total_steps, float beta1, float beta2,
                     float eps, uint32_t seed, float *loss_out);
 
+/* on = 1 (default): crime_mlp_train replays the step as a captured HIP graph; on = 0: it enqueues the
+ * same kernels step by step (the same results; for tools that trace kernel launches). */
+int crime_mlp_set_graph(crime_mlp_t *h, int on);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,3 +110,18 @@ def test_training_converges_and_replays_deterministically():
     print(f"training: loss {la[:10].mean():.3e} -> {la[-50:].mean():.3e}; test mse {np.mean((yt - tt) ** 2):.3e}")
     assert la[-50:].mean() < 0.02 * la[:10].mean()
     assert np.mean((yt - tt) ** 2) < 0.02 * np.var(tt)
+
+
+def test_eager_training_equals_graph_replay():
+    """Steps enqueued one by one (crime_mlp_set_graph(h, 0)) run the same kernels as the replayed graph:
+    bit-identical parameters."""
+    xall, tall = data(8192, 12)
+    p0 = mo.init_params(4)
+    with Mlp(max_batch=1024) as a, Mlp(max_batch=1024) as b:
+        for net in (a, b):
+            net.set_params(p0)
+            net.set_data(xall, tall)
+        b.set_graph(False)
+        a.train(40, 1024, 2e-3, 40, seed=5)
+        b.train(40, 1024, 2e-3, 40, seed=5)
+        assert np.array_equal(a.params(), b.params())
