@@ -5,9 +5,13 @@ A step is one full pass of the hot path (mode fill, 3-D inverse FFT of delta and
 velocity, lognormal HI mass, RSD, sub-cell ang2pix and the (nu, pixel) scatter into zeroed mK maps)
 over the 1024^3 production workload, with every input already resident in HBM.  value = cells/s.
 
-Prints one JSON line.  Extra objects: "roofline" for the dominant kernel (algorithmic bytes per
-launch / its hipEvent-timed duration against 8 TB/s), "cpu_baseline" (the FP64 C oracle, OpenMP on
-the host cores, over a bounded sample of the same workload), "stages" (ms per stage).
+Prints one JSON line.  Extra objects: "roofline" for the dominant kernel (the map kernel, priced against
+the VALU issue rate that bounds it, with its HBM figures beside), "cpu_baseline" (the FP64 C oracle,
+OpenMP on the host cores, over a bounded sample of the same workload), "stages" (ms per stage),
+"pixelize" (samples/s, binned samples/s, global atomics/s), "end_to_end_cli" (the gethi CLI from the
+.ini to the FITS files), and the other two metrics of the headline: "mocks_per_hour_batched" (the
+96-member ensemble in one graph) and "emulator_err_sigma" (the C_l emulator trained and judged at full
+size).  --quick skips the last three.
 """
 import argparse
 import json
@@ -20,7 +24,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of the MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions per second
+# (a wave64 instruction occupies a 16-lane SIMD for 4 cycles; packed FP32 is not used)
+VALU_PEAK_LANE_INSTR = 256 * 4 * 16 * 2.4e9
 WORKLOADS = ("production_1024", "small_fp64_256", "largest_2048")
+METRIC = {"production_1024": "cells/s end-to-end at 1024^3",
+          "small_fp64_256": "cells/s end-to-end at 256^3 (FP64)",
+          "largest_2048": "cells/s end-to-end at 2048^3"}
 
 
 def stage_bytes(n, prec):
@@ -38,16 +48,24 @@ def stage_bytes(n, prec):
 KERNEL_OF_STAGE = {"fill_fft_z": "k_fill_z", "fft_y": "k_fft_y", "fft_x": "k_fft_x", "pixelize": "k_pixelize"}
 
 
-def measured_traffic(workload):
-    """HBM bytes per launch from the committed PMC summary (profiles/hbm_traffic_<workload>.md)."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+def profile_values(workload, key):
+    """Per-launch values of one counter-derived quantity from the committed PMC summaries
+    (lines "<key> <kernel> <value>" in profiles/*_<workload>.md): kernel -> value."""
     out = {}
-    if os.path.exists(path):
-        for line in open(path):
+    pdir = os.path.join(ROOT, "profiles")
+    for name in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
+        if not name.endswith(f"_{workload}.md"):
+            continue
+        for line in open(os.path.join(pdir, name)):
             f = line.split()
-            if len(f) == 3 and f[0] == "traffic_bytes":
+            if len(f) == 3 and f[0] == key:
                 out[f[1]] = float(f[2])
     return out
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch from the committed PMC summary (profiles/hbm_traffic_<workload>.md)."""
+    return profile_values(workload, "traffic_bytes")
 
 
 def cpu_baseline(workload, n_cpu, tmp):
@@ -65,6 +83,100 @@ def cpu_baseline(workload, n_cpu, tmp):
     return {"value": n_cpu ** 3 / dt, "unit": "cells/s", "cores": oracle.num_threads(), "kind": "port",
             "sample": f"{workload} parameters at n_grid={n_cpu} ({n_cpu ** 3} cells, same box, channels, "
                       f"Nside and samples per cell), FP64 C oracle with OpenMP, {dt:.1f} s"}
+
+
+def ensemble_rate(steps=3, streams=4, tune=True):
+    """mocks/hour of the 96-member uneven ensemble in one captured graph (maps only, resident handles),
+    and the plain per-mock rate on the same handles.  Returns a dict."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import crime_amd
+    import crime_amd.configs as cf
+
+    nmem = 96
+    with tempfile.TemporaryDirectory() as tmp:
+        inis = [cf.ensemble_member(tmp, i) for i in range(nmem)]
+        with ThreadPoolExecutor(16) as ex:
+            pars = list(ex.map(crime_amd.Params, inis))
+        hs = [crime_amd.GetHI(p) for p in pars]
+        try:
+            if tune:
+                schedules = {}
+                for h in hs:
+                    key = (h.n, h.n_side)
+                    if key not in schedules:
+                        schedules[key] = h.autotune(reps=2)[0]
+                    h.set_schedule(schedules[key])
+            for h in hs:
+                h.run()
+            t0 = time.perf_counter()
+            for h in hs:
+                h.run()
+            ms_plain = (time.perf_counter() - t0) * 1e3
+            with crime_amd.Batch(hs, n_streams=streams) as b:
+                b.run()
+                ms = min(b.run() for _ in range(steps))
+        finally:
+            for h in hs:
+                h.close()
+    return {"value": nmem * 3600e3 / ms, "unit": "mocks/hour", "ms_per_ensemble": ms, "members": nmem,
+            "plain_resident_mocks_per_hour": nmem * 3600e3 / ms_plain, "speedup_vs_plain_resident": ms_plain / ms,
+            "config": "32x128^3/Nside128 + 32x192^3/Nside256 + 32x256^3/Nside256, 64 channels, one hipGraph"}
+
+
+def emulator_run(steps=20000, log=None):
+    """The C_l emulator at full size (512 x 8 training mocks of 256^3 / Nside 256 / 64 channels, 64 x 16
+    held-out): returns the result dict of crime_amd.emulator.evaluate plus rates."""
+    import crime_amd.emulator as em
+
+    spec = em.MockSpec(n_grid=256, n_side=256, n_nu=64)
+    train_c = em.latin_hypercube(512, seed=1)
+    test_c = em.latin_hypercube(64, seed=2)
+    t0 = time.perf_counter()
+    cls_tr, t_mock = em.simulate(train_c, 8, spec, seed0=100000, log=log)
+    cls_te, _ = em.simulate(test_c, 16, spec, seed0=900000, log=log)
+    t_sim = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    emu, losses = em.train_emulator(train_c, cls_tr, spec, steps=steps, batch=4096, lr0=2e-3, seed=0, log=log)
+    t_train = time.perf_counter() - t1
+    res = em.evaluate(emu, train_c, cls_tr, test_c, cls_te, spec)
+    xq = em.features(em.latin_hypercube(4096, seed=3), spec)
+    emu.predict(xq[:4096])
+    t2 = time.perf_counter()
+    emu.predict(xq)
+    t_eval = (time.perf_counter() - t2) / 4096
+    emu.mlp.close()
+    res.update(ms_per_mock=t_mock * 1e3, us_per_emulated_cosmology=t_eval * 1e6,
+               emulator_rate_vs_mock_rate=t_mock / t_eval, simulation_s=t_sim, training_s=t_train,
+               training_ms_per_step=t_train * 1e3 / steps, final_loss=float(losses[-200:].mean()),
+               gain_vs_nearest_cosmology=res["nearest_cosmology"]["rms_debiased"] /
+               max(res["emulator"]["rms_debiased"], 1e-9))
+    return res
+
+
+def cli_end_to_end(workload, tmp):
+    """Wall time of the gethi CLI from the parameter file to the FITS maps and nu table, in a child process
+    (GPU set-up, host tables, run, maps to the host, FITS writes).  Output into tmp (removed afterwards)."""
+    import subprocess
+
+    import crime_amd.configs as cf
+
+    ini = getattr(cf, workload)(tmp, name=f"cli_{workload}")
+    exe = os.path.join(ROOT, "crime_amd", "_lib", "gethi")
+    t0 = time.perf_counter()
+    r = subprocess.run([exe, ini], capture_output=True, text=True)
+    wall = time.perf_counter() - t0
+    if r.returncode:
+        return {"error": r.stderr[-500:]}
+    out = {"seconds": wall}
+    for line in r.stdout.splitlines():
+        if "maps to the host in" in line:
+            f = line.split()
+            out["ms_maps_to_host"] = float(f[f.index("in") + 1])
+            out["ms_fits_after_copy"] = float(f[f.index("+") + 1])
+    nbytes = sum(os.path.getsize(os.path.join(tmp, f)) for f in os.listdir(tmp) if f.endswith(".fits"))
+    out["fits_bytes"] = nbytes
+    return out
 
 
 def bench_ensemble(args):
@@ -293,6 +405,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=768, help="grid of the CPU-baseline sample (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (scale reference for PMC byte counters)")
+    ap.add_argument("--quick", action="store_true",
+                    help="the map workload only: no ensemble, emulator or CLI run (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the GetHI path on one GPU; multi-GPU slabs are a design note "
@@ -351,6 +465,10 @@ def main():
         maps = gh.maps()
         ms_d2h = (time.perf_counter() - t2) * 1e3
         binned = float(np.asarray(maps, dtype=np.float64).sum())
+        del maps
+        # samples the map kernel bins (an FP64 pass over every sample, after the timed region)
+        _, per_channel = gh.channel_totals()
+        n_binned = int(per_channel.sum())
         gh.close()
 
         keys = ["ms_fill_fft_z", "ms_fft_y", "ms_fft_x", "ms_sigma2", "ms_maps_clear", "ms_pixelize", "ms_total"]
@@ -369,8 +487,46 @@ def main():
             cpu = cpu_baseline(args.workload, args.cpu_n, tmp)
 
         cells = float(n) ** 3
+        ms_pix = launch["pixelize"]
+        atomics = profile_values(args.workload, "global_atomics").get("k_pixelize")
+        valu = profile_values(args.workload, "valu_lane_instr").get("k_pixelize")
+        pixelize = {"ms_per_launch": ms_pix, "samples": cells * d.n_sub, "binned_samples": n_binned,
+                    "samples_per_s": cells * d.n_sub / (ms_pix * 1e-3), "binned_samples_per_s": n_binned / (ms_pix * 1e-3),
+                    "global_atomics_per_launch": atomics,
+                    "global_atomics_per_s": atomics / (ms_pix * 1e-3) if atomics else None,
+                    "counters": f"profiles/*_{args.workload}.md"}
+        if dom == "pixelize" and valu:
+            roof = {"bound": "valu", "kernel": "k_pixelize", "achieved": valu / (ms_pix * 1e-3),
+                    "peak": VALU_PEAK_LANE_INSTR, "unit": "VALU lane-instr/s",
+                    "frac": valu / (ms_pix * 1e-3) / VALU_PEAK_LANE_INSTR,
+                    "traffic": measured_traffic(args.workload).get("k_pixelize"),
+                    "valu_lane_instr_per_launch": valu, "ms_per_launch": ms_pix,
+                    "hbm": {"bytes_per_launch": by["pixelize"], "achieved_GB/s": achieved if dom == "pixelize" else None,
+                            "frac": (achieved / HBM_PEAK_GBS) if dom == "pixelize" else None},
+                    "note": "VALU lane-instructions per launch from the committed SQ_INSTS_VALU profile x 64, "
+                            "divided by the live hipEvent duration; HBM bytes are the algorithmic read of the fields"}
+        else:
+            roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload).get(KERNEL_OF_STAGE[dom]),
+                    "bytes_per_launch": by[dom], "ms_per_launch": launch[dom]}
+        extras = {}
+        if not args.quick and args.workload == "production_1024":
+            print("bench: gethi CLI end to end", file=sys.stderr, flush=True)
+            with tempfile.TemporaryDirectory() as ctmp:
+                cli = cli_end_to_end(args.workload, ctmp)
+            if "seconds" in cli:
+                cli["cells_per_s"] = cells / cli["seconds"]
+            extras["end_to_end_cli"] = cli
+            print("bench: 96-member ensemble", file=sys.stderr, flush=True)
+            ens = ensemble_rate()
+            extras["mocks_per_hour_batched"] = ens["value"]
+            extras["ensemble"] = ens
+            print("bench: emulator (full size)", file=sys.stderr, flush=True)
+            emu = emulator_run(log=lambda m: print(f"bench: {m}", file=sys.stderr, flush=True))
+            extras["emulator_err_sigma"] = emu["emulator"]["rms_debiased"]
+            extras["emulator"] = emu
         out = {
-            "metric": "cells/s end-to-end at 1024^3",
+            "metric": METRIC[args.workload],
             "value": cells / (ms * 1e-3),
             "unit": "cells/s",
             "n_gpus": 1,
@@ -384,20 +540,16 @@ def main():
             "data": "synthetic (Philox Gaussian modes of a BBKS P(k); resident in HBM)",
             "config": {"workload": args.workload, "n_grid": n, "n_side": int(d.n_side), "n_nu": d.n_nu,
                        "samples_per_cell": d.n_sub, "l_box": d.l_box, "seed": d.seed},
-            "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "traffic": measured_traffic(args.workload).get(KERNEL_OF_STAGE[dom]),
-                         "bytes_per_launch": by[dom], "ms_per_launch": launch[dom],
-                         "note": ("pixelize is bound by VALU issue, not HBM: 84% of each SIMD's cycles issue VALU "
-                                  "at 1024^3 (profiles/sq_counters_pixelize_production_1024.md)")
-                         if dom == "pixelize" else None},
+            "roofline": roof,
             "cpu_baseline": cpu,
             "stages": stages,
+            "pixelize": pixelize,
             "autotune": tune,
             "ms_maps_d2h": ms_d2h,
             "cells_per_s_with_d2h": cells / ((ms + ms_d2h) * 1e-3),
             "maps_sum_mK": binned,
         }
+        out.update(extras)
         print(json.dumps(out), flush=True)
 
 

@@ -151,6 +151,10 @@ class GetHI:
         self._check(self._lib.crime_gh_copy_grid(self._h, which, _dptr(out)))
         return out
 
+    def set_seed(self, seed):
+        """A new realisation on this handle: the Philox key of the next field stage (crime_gh_set_seed)."""
+        self._check(self._lib.crime_gh_set_seed(self._h, int(seed) & 0xFFFFFFFF))
+
     def grid_range(self, which, cell0, ncell):
         """Cells [cell0, cell0 + ncell) of a grid (x fastest), as FP64 (crime_gh_copy_grid_range)."""
         out = np.empty(ncell, np.float64)
@@ -166,6 +170,17 @@ class GetHI:
         self._check(self._lib.crime_gh_channel_totals(self._h, int(cell0), int(ncell), _dptr(tot),
                                                       cnt.ctypes.data_as(C.POINTER(C.c_int64))))
         return tot, cnt
+
+    def screen_check(self):
+        """Every sample through the FP32 screen and the FP64 path (crime_gh_screen_check): dict of counts
+        (samples, channel_fallbacks, pixel_fallbacks, mismatches, binned) and the two margins."""
+        c = np.zeros(5, np.int64)
+        eps = np.zeros(2)
+        self._check(self._lib.crime_gh_screen_check(self._h, c.ctypes.data_as(C.POINTER(C.c_int64)), _dptr(eps)))
+        keys = ("samples", "channel_fallbacks", "pixel_fallbacks", "mismatches", "binned")
+        out = {k: int(v) for k, v in zip(keys, c)}
+        out.update(eps_nu_mhz=float(eps[0]), eps_pix=float(eps[1]))
+        return out
 
     def reference_maps(self):
         """FP64 maps scattered sample by sample without the cell screen or the LDS aggregation
@@ -209,12 +224,22 @@ class GetHI:
 
     def autotune(self, reps=3):
         """Time every variant of every stage on this handle and keep the fastest (crime_gh_autotune).
-        Returns (schedule, {stage: [ms of variant 0, 1, 2]}); -1 marks a variant the grid lacks."""
+        Returns (schedule, {stage: [ms of variant 0, 1, 2, 3]}); -1 marks a variant the grid or stage
+        lacks (variant 3, radix-8 plans, exists for the FFT stages only)."""
         s = (C.c_int * 4)()
-        ms = (C.c_double * 12)()
+        nv = _ffi.N_SCHED
+        ms = (C.c_double * (4 * nv))()
         self._check(self._lib.crime_gh_autotune(self._h, int(reps), s, ms))
-        table = {st: [ms[3 * i + v] for v in range(3)] for i, st in enumerate(self.STAGES)}
+        table = {st: [ms[nv * i + v] for v in range(nv)] for i, st in enumerate(self.STAGES)}
         return list(s), table
+
+    def autotune_cached(self, path, reps=3):
+        """The schedule of this handle's kind from the cache file `path`, or the autotuner's on a miss
+        (appended to the file) (crime_gh_autotune_cached).  Returns (schedule, hit)."""
+        s = (C.c_int * 4)()
+        hit = C.c_int()
+        self._check(self._lib.crime_gh_autotune_cached(self._h, os.fsencode(path), int(reps), s, C.byref(hit)))
+        return list(s), bool(hit.value)
 
     def power_spectrum(self, nbins=None, bin_dk=None):
         """Shell-averaged P(k) of the resident Gaussian delta: (k_mean [h/Mpc], P [(Mpc/h)^3], n_modes).

@@ -13,6 +13,7 @@ LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
 
 ABI_VERSION = 2
 NZ = 5001
+N_SCHED = 4   # CRIME_GH_N_SCHED: variants per FFT stage (the map stage has 3)
 FP32, FP64 = 0, 1
 MODES_PHILOX, MODES_INJECTED = 0, 1
 NU_TABLE, NU_UNIFORM = 0, 1
@@ -165,14 +166,26 @@ def gpu_lib():
         lib.crime_gh_get_schedule.argtypes = [H, C.POINTER(C.c_int)]
         lib.crime_gh_set_schedule.restype = C.c_int
         lib.crime_gh_set_schedule.argtypes = [H, C.POINTER(C.c_int)]
+        lib.crime_gh_autotune_cached.restype = C.c_int
+        lib.crime_gh_autotune_cached.argtypes = [H, C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         lib.crime_gh_autotune.restype = C.c_int
         lib.crime_gh_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), _dp]
         lib.crime_gh_power_spectrum.restype = C.c_int
         lib.crime_gh_power_spectrum.argtypes = [H, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int64)]
+        lib.crime_gh_set_seed.restype = C.c_int
+        lib.crime_gh_set_seed.argtypes = [H, C.c_uint32]
+        lib.crime_gh_copy_maps_channels.restype = C.c_int
+        lib.crime_gh_copy_maps_channels.argtypes = [H, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        lib.crime_gh_host_alloc.restype = C.c_int
+        lib.crime_gh_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+        lib.crime_gh_host_free.restype = None
+        lib.crime_gh_host_free.argtypes = [C.c_void_p]
         lib.crime_gh_copy_grid_range.restype = C.c_int
         lib.crime_gh_copy_grid_range.argtypes = [H, C.c_int, C.c_int64, C.c_int64, _dp]
         lib.crime_gh_channel_totals.restype = C.c_int
         lib.crime_gh_channel_totals.argtypes = [H, C.c_int64, C.c_int64, _dp, C.POINTER(C.c_int64)]
+        lib.crime_gh_screen_check.restype = C.c_int
+        lib.crime_gh_screen_check.argtypes = [H, C.POINTER(C.c_int64), _dp]
         lib.crime_gh_reference_maps.restype = C.c_int
         lib.crime_gh_reference_maps.argtypes = [H, _dp]
         lib.crime_gh_device_maps.restype = C.c_int
@@ -246,6 +259,8 @@ def gpu_lib():
         lib.crime_mlp_loss_grad.argtypes = [H, C.c_int, _fp, _fp, _fp, _dp, _fp]
         lib.crime_mlp_set_data.restype = C.c_int
         lib.crime_mlp_set_data.argtypes = [H, C.c_int64, _fp, _fp, _fp]
+        lib.crime_mlp_set_graph.restype = C.c_int
+        lib.crime_mlp_set_graph.argtypes = [H, C.c_int]
         lib.crime_mlp_train.restype = C.c_int
         lib.crime_mlp_train.argtypes = [H, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_float, C.c_float,
                                         C.c_uint32, _fp]

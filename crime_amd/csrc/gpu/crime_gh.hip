@@ -33,6 +33,8 @@ struct crime_gh {
   double *d_edges = nullptr;   // [n_nu+1]
   double *d_prefac = nullptr;  // [n_nu]
   ChanRec *d_crec = nullptr;   // channel records of the frequency buckets
+  ScreenRec *d_srec = nullptr; // FP32 screen records of the same buckets
+  float4 *d_ztab = nullptr;    // FP32 z(r) table of the screen
   void *d_amp = nullptr;       // amplitude per |k|^2
   uint32_t *d_order = nullptr; // map-stage tiles by distance from the observer
   double *d_sub = nullptr;     // [3*n_sub]
@@ -46,6 +48,7 @@ struct crime_gh {
   MapParams map{};
   double norm = 0;
   bool fields_done = false;
+  double eps_nu = 0;           // frequency margin of the FP32 screen (MHz)
   Schedule sched{};            // kernel variants (crime_gh_autotune / crime_gh_set_schedule)
   hipEvent_t ev[8] = {};
   // point sources (allocated by the first crime_gh_mk_psources_maps)
@@ -57,7 +60,6 @@ struct crime_gh {
 };
 
 static thread_local std::string g_create_err;
-
 
 static int set_error(crime_gh *h, int code, const char *fmt, ...)
 {
@@ -133,6 +135,94 @@ static double phi_balance(const crime_gh_desc *d, bool velocity)
   return sqrt(i0 / i2);
 }
 
+// float nearest to x in the direction of +inf (up) or -inf (down): a bound that stays on its side
+static float f32_up(double x)
+{
+  float f = (float)x;
+  return (double)f < x ? nextafterf(f, INFINITY) : f;
+}
+static float f32_down(double x)
+{
+  float f = (float)x;
+  return (double)f > x ? nextafterf(f, -INFINITY) : f;
+}
+
+// Margins and tables of the FP32 screen (MapParams::scr), from bounds on the FP32 operations (u = 2^-23;
+// v_sqrt_f32, v_rcp_f32 and v_rsq_f32 are within 1 ulp, +, *, fma and conversions within 0.5 ulp):
+//  - frequency: |p| = sqrt((float)|p|^2_FP64) within 1.5 u of |p|; the FP32 lerp of z(r) moves z by at most
+//    2 A u + 2 (1 + z) u with A = r dz/dr (the |p| error through the slope, and the rounding of the
+//    table node, fma, RSD shift and 1 + z); rcp, the product and the float NU_21 add 2.5 u relative.  So
+//    |dnu| <= nu u (2 A / (1 + z) + 4.5), taken at the worst table node of the channels' redshift range,
+//    doubled;
+//  - pixel: the angle (input rounding 0.32 u, t = min/max 0.48 u, polynomial and its evaluation 1.2 u,
+//    scalings 0.75 u) within 2.8 u quarter turns; cos(theta) within 3.3 u; sin(theta) and the cap's
+//    square root within 5 u relative; every floor argument of vec2pix is then within 8 N u of the exact
+//    one, whose FP64 evaluation is itself within 1e-12 of it: margin 16 N u.  |cos theta| and the
+//    quadrant angle w keep 8 u from 2/3 and from 0 or 1.
+static int screen_setup(crime_gh *h, const crime_gh_desc *d, const std::vector<double> &edges,
+                        const std::vector<double> &prefac, const std::vector<int> &guess, int n_guess, double g_lo,
+                        double g_inv)
+{
+  const double u = 1.0 / 8388608.0;
+  ScreenParams &c = h->map.scr;
+  c = ScreenParams{};
+  c.on = d->nu_mode == CRIME_GH_NU_TABLE;
+  // frequency margin
+  const double nu_lo = edges[0], nu_hi = edges[d->n_nu];
+  const double zlo = CRIME_GH_NU_21 / nu_hi - 1 - 0.05, zhi = CRIME_GH_NU_21 / nu_lo - 1 + 0.05;
+  double worst = 0;
+  for (int i = 0; i + 1 < CRIME_GH_NZ; i++) {
+    const double z0 = d->z_r2z[i], z1 = d->z_r2z[i + 1];
+    if (z1 < zlo || z0 > zhi) continue;
+    const double A = (z1 - z0) * d->idr * d->r_r2z[i + 1];
+    const double z = z1 > 0 ? z1 : 0;
+    const double nu = CRIME_GH_NU_21 / (1 + (z0 > 0 ? z0 : 0));
+    worst = fmax(worst, nu * u * (2 * A / (1 + z) + 4.5));
+  }
+  const double eps_nu = 2 * fmax(worst, nu_hi * 4.5 * u);
+  // FP32 z(r) table: (r_i, z_i, slope, 0)
+  std::vector<float4> zt(CRIME_GH_NZ - 1);
+  for (int i = 0; i + 1 < CRIME_GH_NZ; i++)
+    zt[i] = make_float4((float)d->r_r2z[i], (float)d->z_r2z[i], (float)((d->z_r2z[i + 1] - d->z_r2z[i]) * d->idr), 0.f);
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_ztab, zt.size() * sizeof(float4)));
+  HIP_OR_FAIL(h, hipMemcpy(h->d_ztab, zt.data(), zt.size() * sizeof(float4), hipMemcpyHostToDevice));
+  // screen records: [edge + eps, next edge - eps), rounded inwards
+  std::vector<ScreenRec> sr(n_guess);
+  for (int j = 0; j < n_guess; j++) {
+    const int ch = guess[j], last = ch == d->n_nu - 1;
+    const double e0 = edges[ch], e1 = edges[ch + 1], e2 = last ? e1 : edges[ch + 2];
+    ScreenRec r{};
+    r.c = ch;
+    r.lo0 = f32_up(e0 + eps_nu);
+    r.hi0 = f32_down(e1 - eps_nu);
+    r.lo1 = f32_up(e1 + eps_nu);
+    r.hi1 = f32_down(e2 - eps_nu);
+    if (last) {
+      r.lo1 = INFINITY;
+      r.hi1 = -INFINITY;
+    }
+    r.pf0 = prefac[ch];
+    r.pf1 = last ? 0.0 : prefac[ch + 1];
+    sr[j] = r;
+  }
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_srec, sr.size() * sizeof(ScreenRec)));
+  HIP_OR_FAIL(h, hipMemcpy(h->d_srec, sr.data(), sr.size() * sizeof(ScreenRec), hipMemcpyHostToDevice));
+  c.ztab = h->d_ztab;
+  c.srec = h->d_srec;
+  c.idr = (float)d->idr;
+  c.r_min = (float)(1e-3 * d->l_box / d->n_grid);
+  c.z_last = (float)d->z_r2z[CRIME_GH_NZ - 1];
+  c.g_lo = (float)g_lo;
+  c.g_inv = (float)g_inv;
+  c.out_lo = f32_down(nu_lo - eps_nu);
+  c.out_hi = f32_up(nu_hi + eps_nu);
+  c.eps_pix = (float)(16 * (double)d->n_side * u);
+  c.eps_ax = (float)(8 * u);
+  c.eps_z = (float)(8 * u);
+  h->eps_nu = eps_nu;
+  return 0;
+}
+
 static int validate(const crime_gh_desc *d)
 {
   if (!d) return set_error(nullptr, CRIME_GH_EINVAL, "descriptor is NULL");
@@ -193,7 +283,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub,
+  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_srec, h->d_ztab, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig, h->d_ps, h->d_ps_count, h->ps_maps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -341,6 +431,10 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   }
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_crec, n_guess * sizeof(ChanRec)));
   HIP_OR_FAIL(h, hipMemcpy(h->d_crec, crec.data(), n_guess * sizeof(ChanRec), hipMemcpyHostToDevice));
+  {
+    const int rc = screen_setup(h, d, edges, prefac, guess, n_guess, g_lo, g_inv);
+    if (rc) return rc;
+  }
 
   if (h->prec) {
     std::vector<cx<double>> tw;
@@ -616,10 +710,20 @@ extern "C" int crime_gh_power_spectrum(crime_gh_t *h, int nbins, double bin_dk, 
   return 0;
 }
 
-// zero n doubles (a kernel, not a memset: no memset node if a caller captures the stream)
 __global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ p, size_t n)
 {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0;
+}
+
+extern "C" int crime_gh_set_seed(crime_gh_t *h, uint32_t seed)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (h->d.mode_source != CRIME_GH_MODES_PHILOX)
+    return set_error(h, CRIME_GH_ESTATE, "set_seed: the modes of this handle are injected");
+  h->d.seed = seed;
+  h->fill.seed = seed;
+  h->fields_done = false;
+  return 0;
 }
 
 extern "C" int crime_gh_get_schedule(const crime_gh_t *h, int *sched)
@@ -637,7 +741,8 @@ extern "C" int crime_gh_set_schedule(crime_gh_t *h, const int *sched)
   if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
   if (!sched) return set_error(h, CRIME_GH_EINVAL, "schedule is NULL");
   for (int i = 0; i < 4; i++)
-    if (sched[i] < 0 || sched[i] >= N_SCHED) return set_error(h, CRIME_GH_EINVAL, "schedule[%d] = %d", i, sched[i]);
+    if (sched[i] < 0 || sched[i] >= (i < 3 ? N_SCHED : N_SCHED_PIX))
+      return set_error(h, CRIME_GH_EINVAL, "schedule[%d] = %d", i, sched[i]);
   const bool tuned = h->n >= TUNE_MIN_N;
   h->sched.fill = tuned ? sched[0] : 0;
   h->sched.y = tuned ? sched[1] : 0;
@@ -682,7 +787,7 @@ extern "C" int crime_gh_autotune(crime_gh_t *h, int reps, int *sched_out, double
   if (rc) return rc;
   h->fields_done = true;
   int best = 0;
-  for (int v = 0; v < N_SCHED; v++) {
+  for (int v = 0; v < N_SCHED_PIX; v++) {
     h->sched.pix = v;
     double t = 1e300;
     for (int r = 0; r < reps; r++) {
@@ -706,6 +811,58 @@ extern "C" int crime_gh_autotune(crime_gh_t *h, int reps, int *sched_out, double
   return 0;
 }
 
+// Key of a handle's kind for the schedule cache: everything the kernel variants' speed depends on
+static std::string schedule_key(const crime_gh *h)
+{
+  hipDeviceProp_t prop;
+  const char *arch = hipGetDeviceProperties(&prop, h->d.device) == hipSuccess ? prop.gcnArchName : "gfx950";
+  char buf[256];
+  snprintf(buf, sizeof(buf), "v%d:%s:N%d:p%d:ns%ld:nu%d:sub%d:vel%d:det%d:mode%d", CRIME_GH_SCHED_VERSION, arch,
+           h->n, h->prec, (long)h->nside, h->n_nu, h->d.n_sub, h->d.velocity_mode, h->d.deterministic, h->d.nu_mode);
+  std::string k = buf;
+  for (char &c : k)
+    if (c == ' ') c = '_';
+  return k;
+}
+
+extern "C" int crime_gh_autotune_cached(crime_gh_t *h, const char *path, int reps, int *sched_out, int *hit)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (!path) return set_error(h, CRIME_GH_EINVAL, "cache path is NULL");
+  const std::string key = schedule_key(h);
+  int sched[4];
+  bool found = false;
+  if (FILE *f = fopen(path, "r")) {
+    char line[512], k[300];
+    int a, b, c, d;
+    while (fgets(line, sizeof(line), f))
+      if (sscanf(line, "%299s %d %d %d %d", k, &a, &b, &c, &d) == 5 && key == k) {
+        sched[0] = a;  // the last entry of a key wins
+        sched[1] = b;
+        sched[2] = c;
+        sched[3] = d;
+        found = true;
+      }
+    fclose(f);
+  }
+  if (hit) *hit = found ? 1 : 0;
+  if (found) {
+    int rc = crime_gh_set_schedule(h, sched);
+    if (rc) return rc;
+  }
+  else {
+    int rc = crime_gh_autotune(h, reps, sched, nullptr);
+    if (rc) return rc;
+    FILE *f = fopen(path, "a");
+    if (!f) return set_error(h, CRIME_GH_EINVAL, "cannot append to the schedule cache %s", path);
+    // one short line per append: appends of concurrent tuners do not interleave
+    fprintf(f, "%s %d %d %d %d\n", key.c_str(), sched[0], sched[1], sched[2], sched[3]);
+    fclose(f);
+  }
+  if (sched_out) crime_gh_get_schedule(h, sched_out);
+  return 0;
+}
+
 extern "C" int crime_gh_copy_maps(crime_gh_t *h, float *out)
 {
   if (!h || !out) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
@@ -718,6 +875,41 @@ extern "C" int crime_gh_copy_maps(crime_gh_t *h, float *out)
   HIP_OR_FAIL(h, hipMemcpy(tmp.data(), h->maps, h->nmaps * sizeof(double), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < h->nmaps; i++) out[i] = (float)tmp[i];
   return 0;
+}
+
+extern "C" int crime_gh_copy_maps_channels(crime_gh_t *h, int c0, int nc, float *out)
+{
+  if (!h || !out) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (c0 < 0 || nc < 1 || c0 + nc > h->n_nu)
+    return set_error(h, CRIME_GH_EINVAL, "channels [%d, %d) outside [0, %d)", c0, c0 + nc, h->n_nu);
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  const size_t n = (size_t)nc * h->npix, off = (size_t)c0 * h->npix;
+  if (!h->prec) {
+    HIP_OR_FAIL(h, hipMemcpy(out, (const float *)h->maps + off, n * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<double> tmp(n);
+  HIP_OR_FAIL(h, hipMemcpy(tmp.data(), (const double *)h->maps + off, n * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; i++) out[i] = (float)tmp[i];
+  return 0;
+}
+
+extern "C" int crime_gh_host_alloc(size_t bytes, void **out)
+{
+  if (!out || !bytes) return set_error(nullptr, CRIME_GH_EINVAL, "host_alloc: NULL or empty");
+  *out = nullptr;
+  hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return set_error(nullptr, CRIME_GH_ENOMEM, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  return 0;
+}
+
+extern "C" void crime_gh_host_free(void *p)
+{
+  if (p) (void)hipHostFree(p);
 }
 
 extern "C" int crime_gh_copy_maps_f64(crime_gh_t *h, double *out)
@@ -873,6 +1065,31 @@ extern "C" int crime_gh_channel_totals(crime_gh_t *h, int64_t cell0, int64_t nce
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return set_error(h, CRIME_GH_EHIP, "channel_totals: %s", hipGetErrorString(e));
+  }
+  return 0;
+}
+
+extern "C" int crime_gh_screen_check(crime_gh_t *h, int64_t *counts, double *eps)
+{
+  if (!h || !counts) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (!h->fields_done) return set_error(h, CRIME_GH_ESTATE, "screen_check before create_d_and_vr_fields");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  if (eps) {
+    eps[0] = h->eps_nu;
+    eps[1] = h->map.scr.eps_pix;
+  }
+  for (int i = 0; i < 5; i++) counts[i] = 0;
+  if (!h->map.scr.on) return 0;
+  unsigned long long *d_c = nullptr;
+  HIP_OR_FAIL(h, hipMalloc((void **)&d_c, 5 * sizeof(unsigned long long)));
+  hipError_t e = h->prec ? launch_screen_check<double>((const cx<double> *)h->F, h->map, d_c, h->stream)
+                         : launch_screen_check<float>((const cx<float> *)h->F, h->map, d_c, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(counts, d_c, 5 * sizeof(int64_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d_c);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(h, CRIME_GH_EHIP, "screen_check: %s", hipGetErrorString(e));
   }
   return 0;
 }

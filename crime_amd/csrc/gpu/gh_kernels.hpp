@@ -32,21 +32,31 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 template <typename T>
 constexpr int emax() { return sizeof(T) == 4 ? 64 : 32; }
 
-// Schedule variants the autotuner chooses between, per stage (variant 0 is the default).  Grids
-// below TUNE_MIN_N have variant 0 only.
-constexpr int N_SCHED = 3;
+// Schedule variants the autotuner chooses between, per stage (variant 0 is the default).  FFT stages:
+// variants 0-2 differ in tiling with radix-16 plans, variant 3 is variant 0's tiling with radix-8 plans
+// (one pass more, smaller butterflies).  The map stage has N_SCHED_PIX variants.  Grids below
+// TUNE_MIN_N have variant 0 only for the FFT stages.
+constexpr int N_SCHED = 4;
+constexpr int N_SCHED_PIX = 3;
+constexpr int CRIME_GH_SCHED_VERSION = 2;  // bump when a variant's meaning changes (schedule cache keys)
 constexpr int TUNE_MIN_N = 128;
 struct Schedule {
   int fill = 0, y = 0, x = 0, pix = 0;
 };
 
+// radix limit and tiling of FFT variant V
+constexpr int fft_maxr(int V) { return V == 3 ? 8 : 16; }
+constexpr int fft_tiling(int V) { return V == 3 ? 0 : V; }
+
 // strided (kz, ky) passes: TX adjacent columns per workgroup, TC threads per column; variants aim at
 // 16, 8 or 4 columns per tile
 template <typename T, int N, int V = 0>
 struct ColCfg {
-  static constexpr int TC = plan_tc(N, cmax(64, N / emax<T>()));
+  static constexpr int MAXR = fft_maxr(V);
+  static constexpr int W = fft_tiling(V);
+  static constexpr int TC = plan_tc(N, cmax(64, N / emax<T>()), MAXR);
   static constexpr int CB = N * (int)sizeof(cx<T>);
-  static constexpr int TXMAX = cmin(V == 0 ? 16 : (V == 1 ? 8 : 4), cpow2_div(N));
+  static constexpr int TXMAX = cmin(W == 0 ? 16 : (W == 1 ? 8 : 4), cpow2_div(N));
   static constexpr int tx_for(int budget)
   {
     int tx = TXMAX;
@@ -65,8 +75,10 @@ struct ColCfg {
 // variants take 32 KB or 128 KB tiles instead
 template <typename T, int N, int V = 0>
 struct FillCfg {
-  static constexpr int TC = ColCfg<T, N>::TC;
-  static constexpr int TX = cmin(ColCfg<T, N>::tx_for(V == 0 ? 65536 : (V == 1 ? 32768 : 131072)), 1024 / TC);
+  static constexpr int MAXR = fft_maxr(V);
+  static constexpr int W = fft_tiling(V);
+  static constexpr int TC = ColCfg<T, N, V>::TC;
+  static constexpr int TX = cmin(ColCfg<T, N, V>::tx_for(W == 0 ? 65536 : (W == 1 ? 32768 : 131072)), 1024 / TC);
   static constexpr int NT = TX * TC;
 };
 
@@ -74,7 +86,9 @@ struct FillCfg {
 // rows (when LDS, threads and N^2 allow)
 template <typename T, int N, int V = 0>
 struct RowCfg {
-  static constexpr int TC = plan_tc(N, cmax(64, N / emax<T>()));
+  static constexpr int MAXR = fft_maxr(V);
+  static constexpr int W = fft_tiling(V);
+  static constexpr int TC = plan_tc(N, cmax(64, N / emax<T>()), MAXR);
   static constexpr int RB = N * (int)sizeof(cx<T>);
   static constexpr int ty_for(int budget)
   {
@@ -84,7 +98,7 @@ struct RowCfg {
   }
   static constexpr int TY0 = ty_for(65536) * RB <= 65536 ? ty_for(65536) : ty_for(131072);
   static constexpr bool CAN2 = 2 * TY0 * RB <= 131072 && 2 * TY0 * TC <= 1024 && (N * N) % (2 * TY0) == 0;
-  static constexpr int TY = V == 1 ? cmax(TY0 / 2, 1) : (V == 2 && CAN2 ? 2 * TY0 : TY0);
+  static constexpr int TY = W == 1 ? cmax(TY0 / 2, 1) : (W == 2 && CAN2 ? 2 * TY0 : TY0);
   static constexpr int NT = TY * TC;
 };
 
@@ -118,6 +132,31 @@ inline int amp_count(int n) { return 3 * (n / 2) * (n / 2) + 1; }
 template <typename T>
 hipError_t launch_amp_table(const FillParams &f, void *tab, hipStream_t s);
 
+// FP32 screen of the per-sample decisions (channel and RING pixel).  Each decision is taken from FP32
+// arithmetic when its argument lies farther than a certified margin from the decision boundary, and
+// from the FP64 path (the reference's arithmetic) otherwise; both give the same integers.
+// Screen record of a frequency bucket: the channel c is decided when lo0 <= nu < hi0, channel c + 1
+// when lo1 <= nu < hi1; the bounds are the channel edges moved inwards by the frequency margin and
+// rounded inwards to float.
+struct __align__(16) ScreenRec {
+  float lo0, hi0, lo1, hi1;
+  double pf0, pf1;  // mK factors of channels c, c + 1 (the FP64 record's values)
+  int c, pad0, pad1, pad2;
+};
+
+struct ScreenParams {
+  int on;                  // 0: every decision in FP64
+  const float4 *ztab;      // [NZ - 1]: (r_i, z_i, (z_{i+1} - z_i) idr, 0)
+  const ScreenRec *srec;   // [n_guess]
+  float idr, r_min;        // table scale; radii below r_min are left to FP64
+  float z_last;            // z of the last table node (z(r) beyond it)
+  float g_lo, g_inv;       // buckets in FP32 (the record check is exact whatever bucket is read)
+  float out_lo, out_hi;    // nu < out_lo or nu >= out_hi: no channel (nu_lo - eps, nu_hi + eps)
+  float eps_pix;           // margin of the pixel floors, in the units of their arguments
+  float eps_ax;            // margin of the quadrant / octant boundaries (w near 0 or 1), quarter turns
+  float eps_z;             // margin of |cos theta| around 2/3
+};
+
 struct MapParams {
   int n;
   double dx, obs[3];
@@ -149,6 +188,7 @@ struct MapParams {
   unsigned long long *fxmaps;
   double *fxs;
   double *mass_part;
+  ScreenParams scr;
 };
 
 constexpr int FX_PARTS = 2048;  // blocks of the fixed-order mass reduction
@@ -213,6 +253,11 @@ hipError_t launch_channel_totals(const cx<T> *F, const MapParams &m, int64_t cel
                                  unsigned long long *cnt, hipStream_t s);
 template <typename T>
 hipError_t launch_reference_maps(const cx<T> *F, const MapParams &m, double *ref, hipStream_t s);
+// every sample of the box through the screened and the FP64 decisions: counts[0] samples, [1] channel
+// decisions left to FP64, [2] pixel decisions left to FP64 (binned samples), [3] samples whose screened
+// (channel, pixel) differ from the FP64 ones, [4] binned samples (counts cleared here)
+template <typename T>
+hipError_t launch_screen_check(const cx<T> *F, const MapParams &m, unsigned long long *counts, hipStream_t s);
 template <typename T>
 hipError_t launch_grid_get(const cx<T> *F, size_t ncell, int which, double *out, hipStream_t s);
 template <typename T>

@@ -49,10 +49,10 @@ __device__ __forceinline__ void mode_uniforms(const FillParams &f, size_t idx, d
 // rng_delta_gauss (src/common.c:154-164), phase 2 pi u1 and modulus sqrt(-ln(1-u2)).  FP64 grids
 // evaluate it in FP64; FP32 grids evaluate the logarithm, root and sin/cos in FP32 from the exact
 // FP64 value of 1-u2 (relative error ~1e-7 per mode, the precision the mode is stored in).
-template <typename T>
+template <typename T, int N>
 __device__ __forceinline__ cx<T> unit_mode(const FillParams &f, int kx, int ky, int kz)
 {
-  size_t idx = (size_t)kx + (size_t)(f.n / 2 + 1) * ((size_t)ky + (size_t)f.n * (size_t)kz);
+  size_t idx = (size_t)kx + (size_t)(N / 2 + 1) * ((size_t)ky + (size_t)N * (size_t)kz);
   if (f.mode_source == 1) return {(T)f.unit_modes[2 * idx], (T)f.unit_modes[2 * idx + 1]};
   double u1, u2;
   mode_uniforms(f, idx, u1, u2);
@@ -91,12 +91,13 @@ __global__ __launch_bounds__(256) void k_amp_table(FillParams f, AmpEntry<T> *__
   else tab[m] = (float)amp;
 }
 
-// F = delta + i phi_scale phi at full-grid position (kx, ky, kz), kx in [0, N)
-template <typename T>
+// F = delta + i phi_scale phi at full-grid position (kx, ky, kz), kx in [0, N) (N a compile-time
+// constant: the mirror indices are masks or constant divisions, not runtime integer divisions)
+template <typename T, int N>
 __device__ __forceinline__ cx<T> fill_element(const FillParams &f, const AmpEntry<T> *__restrict__ amp, int kx,
                                               int ky, int kz)
 {
-  const int n = f.n, h = n / 2;
+  constexpr int n = N, h = N / 2;
   if ((kx | ky | kz) == 0) return {(T)0, (T)0};
   // |k| components with the reference's +Nyquist convention (fourier.c:264-283); only squares matter
   int ax = kx <= h ? kx : n - kx;
@@ -106,20 +107,22 @@ __device__ __forceinline__ cx<T> fill_element(const FillParams &f, const AmpEntr
   cx<T> ab;
   if constexpr (sizeof(T) == 8) ab = amp[m2];
   else {
+    // psi = psi_c / m2 through the hardware reciprocal (1 ulp): FP32 fields carry ~1e-7 relative error
+    // anyway, and a correctly rounded division costs ten instructions per element
     float a = amp[m2];
-    ab = {a, a * ((float)f.psi_c / (float)m2)};
+    ab = {a, a * ((float)f.psi_c * __builtin_amdgcn_rcpf((float)m2))};
   }
   int myn = (n - ky) % n, mzn = (n - kz) % n;  // -k in y and z
   cx<T> g;
   if (kx > 0 && kx < h) {
-    g = unit_mode<T>(f, kx, ky, kz);
+    g = unit_mode<T, N>(f, kx, ky, kz);
   }
   else if (kx > h) {
-    cx<T> m = unit_mode<T>(f, n - kx, myn, mzn);
+    cx<T> m = unit_mode<T, N>(f, n - kx, myn, mzn);
     g = {m.x, -m.y};
   }
   else {  // kx == 0 or kx == N/2: Hermitian part of the plane
-    cx<T> a = unit_mode<T>(f, kx, ky, kz), b = unit_mode<T>(f, kx, myn, mzn);
+    cx<T> a = unit_mode<T, N>(f, kx, ky, kz), b = unit_mode<T, N>(f, kx, myn, mzn);
     g = {(T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)};
   }
   if (f.vmode == 0) {
@@ -158,12 +161,12 @@ __global__ __launch_bounds__((FillCfg<T, N, V>::NT)) void k_fill_z(cx<T> *__rest
   cx<T> *col = F + (size_t)kx + (size_t)N * ky;
   // the generator is too large to unroll into the register tile: stage the modes in LDS first
   const AmpEntry<T> *amp = (const AmpEntry<T> *)f.amp;
-  for (int e = t; e < N; e += C::TC) lds[e * C::TX + c] = fill_element<T>(f, amp, kx, ky, e);
+  for (int e = t; e < N; e += C::TC) lds[e * C::TX + c] = fill_element<T, N>(f, amp, kx, ky, e);
   __syncthreads();
   auto load = [&](int e) { return lds[e * C::TX + c]; };
   auto store = [&](int e, cx<T> v) { col[(size_t)e * n2] = v; };
   auto slot = [&](int e) -> cx<T> & { return lds[e * C::TX + c]; };
-  Stockham<T, N, C::TC>::template run<true>(t, load, store, slot, tw);
+  Stockham<T, N, C::TC, C::MAXR>::template run<true>(t, load, store, slot, tw);
 }
 
 template <typename T, int N, int V>
@@ -181,7 +184,7 @@ __global__ __launch_bounds__((ColCfg<T, N, V>::NT)) void k_fft_y(cx<T> *__restri
   auto load = [&](int e) { return col[(size_t)e * N]; };
   auto store = [&](int e, cx<T> v) { col[(size_t)e * N] = v; };
   auto slot = [&](int e) -> cx<T> & { return lds[e * C::TX + c]; };
-  Stockham<T, N, C::TC>::run(t, load, store, slot, tw);
+  Stockham<T, N, C::TC, C::MAXR>::run(t, load, store, slot, tw);
 }
 
 template <typename T, int N, int V>
@@ -206,7 +209,7 @@ __global__ __launch_bounds__((RowCfg<T, N, V>::NT)) void k_fft_x(cx<T> *__restri
     line[e] = o;
   };
   auto slot = [&](int e) -> cx<T> & { return lds[b * N + e]; };
-  Stockham<T, N, C::TC>::run(t, load, store, slot, tw);
+  Stockham<T, N, C::TC, C::MAXR>::run(t, load, store, slot, tw);
   red[threadIdx.x] = s1;
   red[C::NT + threadIdx.x] = s2;
   __syncthreads();
@@ -376,6 +379,7 @@ static void fill_sched(cx<T> *F, const FillParams &f, const cx<T> *tw, hipStream
   constexpr bool tuned = N >= TUNE_MIN_N;
   if (tuned && sc.fill == 1) launch_fill_v<T, N, tuned ? 1 : 0>(F, f, tw, s);
   else if (tuned && sc.fill == 2) launch_fill_v<T, N, tuned ? 2 : 0>(F, f, tw, s);
+  else if (tuned && sc.fill == 3) launch_fill_v<T, N, tuned ? 3 : 0>(F, f, tw, s);
   else launch_fill_v<T, N, 0>(F, f, tw, s);
 }
 
@@ -385,6 +389,7 @@ static void y_sched(cx<T> *F, const cx<T> *tw, hipStream_t s, const Schedule &sc
   constexpr bool tuned = N >= TUNE_MIN_N;
   if (tuned && sc.y == 1) launch_y_v<T, N, tuned ? 1 : 0>(F, tw, s);
   else if (tuned && sc.y == 2) launch_y_v<T, N, tuned ? 2 : 0>(F, tw, s);
+  else if (tuned && sc.y == 3) launch_y_v<T, N, tuned ? 3 : 0>(F, tw, s);
   else launch_y_v<T, N, 0>(F, tw, s);
 }
 
@@ -395,6 +400,7 @@ static int x_sched(cx<T> *F, const cx<T> *tw, double nre, double nim, double *pa
   constexpr bool tuned = N >= TUNE_MIN_N;
   if (tuned && sc.x == 1) return launch_x_v<T, N, tuned ? 1 : 0>(F, tw, nre, nim, partials, s);
   if (tuned && sc.x == 2) return launch_x_v<T, N, tuned ? 2 : 0>(F, tw, nre, nim, partials, s);
+  if (tuned && sc.x == 3) return launch_x_v<T, N, tuned ? 3 : 0>(F, tw, nre, nim, partials, s);
   return launch_x_v<T, N, 0>(F, tw, nre, nim, partials, s);
 }
 

@@ -148,6 +148,161 @@ __device__ __forceinline__ int sample_bin(const MapParams &m, const CellState &s
   return m.nu_mode == 1 ? sample_bin_t<false>(m, s, isub, ipix, pf) : sample_bin_t<true>(m, s, isub, ipix, pf);
 }
 
+// ---------------------------------------------------------------------------------------------
+// FP32 screen of the per-sample decisions.  A decision (channel; jp, jm, ip of the RING pixel) is read
+// off FP32 arithmetic when its argument lies farther than a margin from the integer or edge that would
+// change it; the margins (ScreenParams, set by the host from the error bounds of the FP32 operations)
+// exceed the difference between the FP32 argument and the exact one, and the FP64 path's own
+// difference from the exact one is far below them, so an accepted decision is the FP64 path's.  Any
+// decision that misses its margin is taken by the FP64 path of sample_pre/sample_post/vec2pix_ring.
+// ---------------------------------------------------------------------------------------------
+struct SampleS {
+  double x, y, z;         // position relative to the observer (the reference's FP64 sums)
+  float xf, yf, zf, rf;   // the same in FP32, and |p| from them
+  int inu;                // channel, -1 none, -2 undecided
+  double pf;              // mK factor of the channel
+};
+
+// atan(t) on [0, 1] as t P(t^2), P of degree 7 fitted to atan in absolute error; evaluated with FMAs the
+// result is within 1.4e-7 of atan(t) for every float t in [0, 1]
+__device__ __forceinline__ float atan01(float t)
+{
+  const float s = t * t;
+  float p = -0.004054564982652664f;
+  p = fmaf(p, s, 0.021862955763936043f);
+  p = fmaf(p, s, -0.055912334471940994f);
+  p = fmaf(p, s, 0.0964219868183136f);
+  p = fmaf(p, s, -0.1390863060951233f);
+  p = fmaf(p, s, 0.19946566224098206f);
+  p = fmaf(p, s, -0.33329859375953674f);
+  p = fmaf(p, s, 0.9999993443489075f);
+  return p * t;
+}
+
+// position, |p| and the channel from FP32 (TABLE mode): q.inu = channel, -1 (no channel) or -2 (left to FP64)
+__device__ __forceinline__ void screen_channel(const MapParams &m, const CellState &s, float dzf, int isub, SampleS &q)
+{
+  const ScreenParams &c = m.scr;
+  const const_double *so = (const const_double *)m.sub_off;
+  q.x = s.x0 + so[3 * isub];
+  q.y = s.y0 + so[3 * isub + 1];
+  q.z = s.z0 + so[3 * isub + 2];
+  q.xf = (float)q.x;
+  q.yf = (float)q.y;
+  q.zf = (float)q.z;
+  // |p|^2 in FP64 (exact to 1e-16), its root in FP32: |p| within 1.25 ulp
+  q.rf = __builtin_amdgcn_sqrtf((float)(q.x * q.x + q.y * q.y + q.z * q.z));
+  const int ir = (int)fminf(fmaxf(q.rf * c.idr, 0.0f), (float)(5001 - 2));
+  const float4 e = c.ztab[ir];
+  // past the last node z(r) is the last entry, as in tab_lerp (the lerp is continuous there)
+  const float zr = fminf(fmaf(e.z, q.rf - e.x, e.y), c.z_last) + dzf;
+  const float nu = 1420.40575177f * __builtin_amdgcn_rcpf(1.0f + zr);
+  const int b = (int)fminf(fmaxf((nu - c.g_lo) * c.g_inv, 0.0f), (float)(m.n_guess - 1));
+  const ScreenRec r = c.srec[b];
+  int inu = -2;
+  if (nu >= r.lo0 && nu < r.hi0) {
+    inu = r.c;
+    q.pf = r.pf0;
+  }
+  else if (nu >= r.lo1 && nu < r.hi1) {
+    inu = r.c + 1;
+    q.pf = r.pf1;
+  }
+  else if (nu < c.out_lo || nu >= c.out_hi) {
+    inu = -1;
+  }
+  if (!(q.rf > c.r_min)) inu = -2;  // near the observer
+  q.inu = inu;
+}
+
+// RING pixel from FP32, or -2 when a margin is not met (see vec2pix_ring for the FP64 form)
+__device__ __forceinline__ int64_t screen_pixel(const MapParams &m, const SampleS &q)
+{
+  const ScreenParams &c = m.scr;
+  const int nside = (int)m.nside;
+  const float N = (float)nside;
+  const float cth = q.zf * __builtin_amdgcn_rcpf(q.rf);
+  const float za = fabsf(cth);
+  const float ax = fabsf(q.xf), ay = fabsf(q.yf);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  // on a coordinate plane atan2's signed-zero conventions decide: FP64
+  if (!(mn > 0.0f) || fabsf(za - (2.0f / 3.0f)) < c.eps_z) return -2;
+  const float u = atan01(mn * __builtin_amdgcn_rcpf(mx)) * 0.63661977236758134f;  // quarter turns, [0, 0.5]
+  const float w = ay > ax ? 1.0f - u : u;  // angle of (|x|, |y|) in quarter turns, [0, 1]
+  if (w < c.eps_ax || w > 1.0f - c.eps_ax) return -2;
+  // tt = phi / (pi/2) in [0, 4) = B + sw
+  const bool xneg = q.xf < 0.0f, yneg = q.yf < 0.0f;
+  const int B = xneg ? 2 : (yneg ? 4 : 0);
+  const float sw = xneg != yneg ? -w : w;
+  const float eps = c.eps_pix;
+  if (za < 2.0f / 3.0f) {
+    // jp, jm = floor(N (0.5 + tt) -+ 0.75 N cth) = K0 + floor(h + N sw -+ 0.75 N cth), N (0.5 + B) = K0 + h
+    const int twoK = nside * (1 + 2 * B);
+    const float h = (twoK & 1) ? 0.5f : 0.0f;
+    const float nw = N * sw, nc = 0.75f * N * cth;
+    const float v1 = (nw - nc) + h, v2 = (nw + nc) + h;
+    const float f1 = floorf(v1), f2 = floorf(v2);
+    const float d1 = v1 - f1, d2 = v2 - f2;
+    if (d1 < eps || d1 > 1.0f - eps || d2 < eps || d2 > 1.0f - eps) return -2;
+    const int jp = (twoK >> 1) + (int)f1, jm = (twoK >> 1) + (int)f2;
+    const int ir = nside + 1 + jp - jm;
+    const int kshift = 1 - (ir & 1);
+    int ip = (jp + jm - nside + kshift + 1) / 2;
+    ip = imod_small(ip, 4 * nside);
+    return (int64_t)(nside * (nside - 1) * 2 + (ir - 1) * 4 * nside + ip);
+  }
+  // polar caps: tp = tt - (int)tt, jp = floor(tp tmp), jm = floor((1 - tp) tmp), tmp = N sin(theta) sqrt(3/(1+za))
+  const float tp = sw > 0.0f ? w : 1.0f - w;
+  const float sth = __builtin_amdgcn_sqrtf(fmaf(q.xf, q.xf, q.yf * q.yf)) * __builtin_amdgcn_rcpf(q.rf);
+  const float tmp = N * sth * __builtin_amdgcn_rsqf((1.0f + za) * (1.0f / 3.0f));
+  const float p1 = tp * tmp, p2 = (1.0f - tp) * tmp;
+  const float g1 = floorf(p1), g2 = floorf(p2);
+  const float d1 = p1 - g1, d2 = p2 - g2;
+  if (d1 < eps || d1 > 1.0f - eps || d2 < eps || d2 > 1.0f - eps) return -2;
+  const int ir = (int)g1 + (int)g2 + 1;
+  // ip = floor(tt ir) = B ir + floor(sw ir)
+  const float p3 = sw * (float)ir;
+  const float g3 = floorf(p3);
+  const float d3 = p3 - g3;
+  if (d3 < eps || d3 > 1.0f - eps) return -2;
+  int ip = B * ir + (int)g3;
+  ip = imod_small(ip, 4 * ir);
+  if (cth > 0.0f) return (int64_t)(2 * ir * (ir - 1) + ip);
+  return (int64_t)(12 * nside * nside - 2 * ir * (ir + 1) + ip);
+}
+
+// finish a screened sample: undecided channels take the whole FP64 path, undecided pixels the FP64
+// vec2pix_ring; returns the channel (-1: not binned) with its pixel in ipix and mK factor in q.pf
+__device__ __forceinline__ int resolve_sample(const MapParams &m, const CellState &s, int isub, SampleS &q,
+                                              int64_t &ipix)
+{
+  if (q.inu == -2) {
+    SamplePre p;
+    sample_pre<true>(m, s, isub, p);
+    q.inu = sample_post<true>(m, p, ipix, q.pf);
+    return q.inu;
+  }
+  ipix = -1;
+  if (q.inu < 0) return -1;
+  ipix = screen_pixel(m, q);
+  if (ipix == -2) ipix = vec2pix_ring(m.nside, q.x, q.y, q.z, sqrt(q.x * q.x + q.y * q.y + q.z * q.z));
+  return q.inu;
+}
+
+// the channel and pixel every map kernel uses: screened in TABLE mode when the screen is on
+__device__ __forceinline__ int sample_decide(const MapParams &m, const CellState &s, float dzf, int isub, int64_t &ipix,
+                                             double &pf)
+{
+  if (m.nu_mode == 0 && m.scr.on) {
+    SampleS q;
+    screen_channel(m, s, dzf, isub, q);
+    const int inu = resolve_sample(m, s, isub, q, ipix);
+    pf = q.pf;
+    return inu;
+  }
+  return sample_bin(m, s, isub, ipix, pf);
+}
+
 // false when no sample of the cell can land in a channel (conservative)
 __device__ __forceinline__ bool cell_may_bin(const MapParams &m, const CellState &s)
 {
@@ -252,7 +407,29 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
                           vmaps);
       }
     };
-    if (m.nu_mode == 0) {
+    if (m.nu_mode == 0 && m.scr.on) {
+      // two samples per iteration, screened: both channel loads in flight before either is resolved
+      const float dzf = (float)s.dz;
+      int isub = 0;
+      for (; isub + 1 < m.n_sub; isub += 2) {
+        SampleS qa, qb;
+        screen_channel(m, s, dzf, isub, qa);
+        screen_channel(m, s, dzf, isub + 1, qb);
+        int64_t pa, pb;
+        const int ia = resolve_sample(m, s, isub, qa, pa);
+        const int ib = resolve_sample(m, s, isub + 1, qb, pb);
+        add(ia, pa, qa.pf);
+        add(ib, pb, qb.pf);
+      }
+      if (isub < m.n_sub) {
+        SampleS q;
+        screen_channel(m, s, dzf, isub, q);
+        int64_t ipix;
+        const int inu = resolve_sample(m, s, isub, q, ipix);
+        add(inu, ipix, q.pf);
+      }
+    }
+    else if (m.nu_mode == 0) {
       // two samples per iteration: their loads and FP64 chains overlap (62.4 against 64.7 ms at 1024^3)
       int isub = 0;
       for (; isub + 1 < m.n_sub; isub += 2) {
@@ -618,10 +795,11 @@ __global__ __launch_bounds__(256) void k_sample_indices(const cx<T> *__restrict_
     CellState s;
     cell_velocity<T>(m, F, ix, iy, iz, s);
     cell_hi(m, (double)F[cell].x, m.sig[0], s);
+    const float dzf = (float)s.dz;
     for (int isub = 0; isub < m.n_sub; isub++) {
       int64_t ipix;
       double pf;
-      int inu = sample_bin(m, s, isub, ipix, pf);
+      int inu = sample_decide(m, s, dzf, isub, ipix, pf);
       inu_out[c * m.n_sub + isub] = inu;
       ipix_out[c * m.n_sub + isub] = ipix;
     }
@@ -678,6 +856,48 @@ __global__ __launch_bounds__(256) void k_parity_scatter(const cx<T> *__restrict_
       }
     }
   }
+}
+
+// every sample through sample_bin (FP64) and the screen: counts of samples, decisions left to FP64 and
+// mismatches (see launch_screen_check)
+template <typename T>
+__global__ __launch_bounds__(256) void k_screen_check(const cx<T> *__restrict__ F, MapParams m,
+                                                      unsigned long long *__restrict__ counts)
+{
+  __shared__ unsigned long long red[5];
+  if (threadIdx.x < 5) red[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long nsmp = 0, nch = 0, npix = 0, nbad = 0, nbin = 0;
+  const size_t ncell = (size_t)m.n * m.n * m.n;
+  for (size_t cell = (size_t)blockIdx.x * 256 + threadIdx.x; cell < ncell; cell += (size_t)gridDim.x * 256) {
+    int ix, iy, iz;
+    cell_coords<T>(m, cell, ix, iy, iz);
+    CellState s;
+    cell_velocity<T>(m, F, ix, iy, iz, s);
+    cell_hi(m, (double)F[cell].x, m.sig[0], s);
+    const float dzf = (float)s.dz;
+    for (int isub = 0; isub < m.n_sub; isub++) {
+      int64_t p64;
+      double pf64;
+      const int i64 = sample_bin(m, s, isub, p64, pf64);
+      SampleS q;
+      screen_channel(m, s, dzf, isub, q);
+      nch += q.inu == -2;
+      if (q.inu >= 0 && screen_pixel(m, q) == -2) npix++;
+      int64_t pq;
+      const int iq = resolve_sample(m, s, isub, q, pq);
+      nsmp++;
+      nbin += i64 >= 0;
+      nbad += (iq != i64) || (i64 >= 0 && (pq != p64 || q.pf != pf64));
+    }
+  }
+  atomicAdd(&red[0], nsmp);
+  atomicAdd(&red[1], nch);
+  atomicAdd(&red[2], npix);
+  atomicAdd(&red[3], nbad);
+  atomicAdd(&red[4], nbin);
+  __syncthreads();
+  if (threadIdx.x < 5) atomicAdd(counts + threadIdx.x, red[threadIdx.x]);
 }
 
 template <typename T>
@@ -758,6 +978,15 @@ hipError_t launch_channel_totals(const cx<T> *F, const MapParams &m, int64_t cel
 }
 
 template <typename T>
+hipError_t launch_screen_check(const cx<T> *F, const MapParams &m, unsigned long long *counts, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(1), dim3(256), 0, s, counts, (size_t)5);
+  const size_t ncell = (size_t)m.n * m.n * m.n;
+  hipLaunchKernelGGL(k_screen_check<T>, dim3(nblk(ncell)), dim3(256), 0, s, F, m, counts);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_reference_maps(const cx<T> *F, const MapParams &m, double *ref, hipStream_t s)
 {
   const int64_t ncell = (int64_t)m.n * m.n * m.n;
@@ -827,6 +1056,7 @@ hipError_t launch_ps_expand(const double *A, const double *B, const double *c1, 
   template hipError_t launch_channel_totals<T>(const cx<T> *, const MapParams &, int64_t, int64_t, double *, \
                                                unsigned long long *, hipStream_t);                      \
   template hipError_t launch_reference_maps<T>(const cx<T> *, const MapParams &, double *, hipStream_t);  \
+  template hipError_t launch_screen_check<T>(const cx<T> *, const MapParams &, unsigned long long *, hipStream_t); \
   template hipError_t launch_grid_get<T>(const cx<T> *, size_t, int, double *, hipStream_t);          \
   template hipError_t launch_grid_set<T>(cx<T> *, size_t, int, const double *, hipStream_t);          \
   template hipError_t launch_psources<T>(const cx<T> *, const MapParams &, const PsParams &, hipStream_t); \

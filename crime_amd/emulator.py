@@ -9,8 +9,10 @@ two).  Network and training run on the GPU through include/crime_mlp.h
 produces the training targets and stays the method the emulator is judged against.
 
 A "mock" here is one GetHI run: Philox Gaussian field of the cosmology's BBKS P(k), lognormal HI with a
-constant bias b, RSD, pixelisation into the frequency channels; its C_l are measured per channel pair and
-averaged in logarithmic l bins.
+constant bias b, RSD, pixelisation into the frequency channels (deterministic fixed-point scatter, so
+every mock is bit-reproducible); its C_l are measured per channel pair and averaged in logarithmic l
+bins.  Training is deterministic too (fixed-order split-K gradients, Philox minibatches): the whole
+pipeline gives the same network on every run.
 """
 import ctypes as C
 import math
@@ -81,6 +83,10 @@ class Mlp:
         w = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
         self._check(self._lib.crime_mlp_set_data(self._h, x.shape[0], _fp(x), _fp(t), None if w is None else _fp(w)))
 
+    def set_graph(self, on):
+        """Replay training steps as a captured graph (default) or enqueue them step by step."""
+        self._check(self._lib.crime_mlp_set_graph(self._h, 1 if on else 0))
+
     def train(self, steps, batch, lr0, total_steps, seed=0, beta1=0.9, beta2=0.999, eps=1e-8):
         loss = np.empty(steps, np.float32)
         self._check(self._lib.crime_mlp_train(self._h, steps, batch, lr0, total_steps, beta1, beta2, eps, seed,
@@ -146,17 +152,18 @@ class MockSpec:
 
 
 def _params_for(cosmo, spec, tmp, name):
+    # deterministic scatter: the training targets (and so the trained network) are the same on every run
     om, s8, ns, h, b = cosmo
     ini = cf.make(tmp, name, n_grid=spec.n_grid, n_side=spec.n_side, n_nu=spec.n_nu, nu_lo=spec.nu_lo,
                   nu_hi=spec.nu_hi, l_box=-1.0, seed=1, omega_m=om, sigma_8=s8, n_s=ns, h=h, precision="single",
-                  bias_model="const", bias_b0=b)
+                  bias_model="const", bias_b0=b, extra=dict(deterministic=1))
     return Params(ini)
 
 
 def simulate(cosmos, nseeds, spec, seed0, log=None):
     """Binned C_l of every (cosmology, realisation): array (ncosmo, nseeds, npairs, nbins), and the seconds
-    per mock (GetHI set-up on the device, run and spectra; host parameter set-up excluded).  Realisation j
-    of cosmology i uses the Philox seed seed0 + i nseeds + j."""
+    per mock (the cosmology's device set-up shared by its realisations, runs and spectra; host parameter
+    set-up excluded).  Realisation j of cosmology i uses the Philox seed seed0 + i nseeds + j."""
     ncos = len(cosmos)
     out = np.empty((ncos, nseeds, len(spec.pairs), spec.nbins))
     t_mock = 0.0
@@ -164,13 +171,15 @@ def simulate(cosmos, nseeds, spec, seed0, log=None):
         pars = list(ex.map(lambda i: _params_for(cosmos[i], spec, tmp, f"c{i:04d}"), range(ncos)))
         with Sht(spec.n_side, spec.lmax, max_maps=spec.n_nu) as sh:
             for i, par in enumerate(pars):
-                for j in range(nseeds):
-                    t0 = time.perf_counter()
-                    with GetHI(par, seed=int(seed0 + i * nseeds + j)) as gh:
+                # one device handle per cosmology; each realisation re-keys the mode fill
+                t0 = time.perf_counter()
+                with GetHI(par, seed=int(seed0 + i * nseeds)) as gh:
+                    for j in range(nseeds):
+                        gh.set_seed(seed0 + i * nseeds + j)
                         gh.run()
                         cl = sh.cls_of(gh, pairs=spec.pairs)
-                    t_mock += time.perf_counter() - t0
-                    out[i, j] = spec.bin(cl)
+                        out[i, j] = spec.bin(cl)
+                t_mock += time.perf_counter() - t0
                 par.close()
                 if log and (i + 1) % max(1, ncos // 8) == 0:
                     log(f"simulated {i + 1}/{ncos} cosmologies")

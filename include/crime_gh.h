@@ -193,6 +193,15 @@ int crime_gh_run(crime_gh_t *h, crime_gh_stats *st);
 int crime_gh_copy_maps(crime_gh_t *h, float *maps_out);
 int crime_gh_copy_maps_f64(crime_gh_t *h, double *maps_out);
 
+/* Channels [c0, c0 + nc) of the maps as float into out[nc * N_pix] (no counterpart in the reference;
+ * the pieces of write_maps' input, src/io_gh.c:109-131, so that a caller can write finished channels
+ * while later ones are copied).  With out from crime_gh_host_alloc the copy runs at the pinned rate. */
+int crime_gh_copy_maps_channels(crime_gh_t *h, int c0, int nc, float *out);
+
+/* Page-locked host memory (hipHostMalloc) for the copies above; free with crime_gh_host_free. */
+int crime_gh_host_alloc(size_t bytes, void **out);
+void crime_gh_host_free(void *p);
+
 /* sigma^2 and mean of the Gaussian field from the last create_d_and_vr_fields. */
 int crime_gh_get_sigma2(crime_gh_t *h, double *sigma2, double *mean);
 
@@ -220,6 +229,15 @@ int crime_gh_sample_indices(crime_gh_t *h, int64_t cell0, int64_t ncell, int32_t
 int crime_gh_channel_totals(crime_gh_t *h, int64_t cell0, int64_t ncell, double *tot, int64_t *count);
 int crime_gh_reference_maps(crime_gh_t *h, double *out);
 
+/* Check of the map stage's FP32 screen (no counterpart in the reference).  The map kernels take each
+ * sample's channel and pixel from FP32 arithmetic wherever the argument of the decision lies farther
+ * than a certified margin from the decision boundary, and from the FP64 path (the arithmetic of
+ * crime_gh_sample_indices' reference form) otherwise.  This call runs every sample of the box through
+ * both: counts[0] samples, [1] channel decisions the screen left to FP64, [2] pixel decisions it left
+ * to FP64, [3] samples whose (channel, pixel, mK factor) differ between the two (must be 0), [4] binned
+ * samples.  eps (may be NULL) receives the frequency margin (MHz) and the pixel margin. */
+int crime_gh_screen_check(crime_gh_t *h, int64_t *counts, double *eps);
+
 /* Power spectrum of the resident Gaussian field delta (no counterpart in the reference): a forward
  * 3-D FFT of the grid and |delta_k|^2 L^3/N^6 averaged over shells |k| in [i, i+1) * bin_dk (h/Mpc),
  * i < nbins <= 2048.  Outputs per shell: mean |k| (h/Mpc), mean P (Mpc/h)^3 and the number of grid
@@ -227,20 +245,35 @@ int crime_gh_reference_maps(crime_gh_t *h, double *out);
  * crime_gh_create_d_and_vr_fields must run again before crime_gh_mk_T_maps. */
 int crime_gh_power_spectrum(crime_gh_t *h, int nbins, double bin_dk, double *k_mean, double *pk, int64_t *nmodes);
 
+/* A new realisation on the same handle (no counterpart in the reference, which reads the seed once,
+ * src/io_gh.c:239): the Philox key of the mode fill becomes `seed`; the next field stage draws the new
+ * modes.  Batches capture the seed at creation.  ESTATE for handles with injected modes. */
+int crime_gh_set_seed(crime_gh_t *h, uint32_t seed);
+
 /* Kernel schedule (no counterpart in the reference): sched[4] = variants of the fill+kz pass, the ky
- * pass, the kx pass and the map-stage aggregation table, each in [0, 3); 0 is the default.  The three
- * FFT variants differ only in tiling (bit-identical fields); the map variants in the size of the LDS
- * aggregation table (2048, 1024, 4096 bins: the atomic order of the FP32 sums changes).  Grids below
- * N = 128 have one FFT variant (their entries read back as 0). */
+ * pass and the kx pass, each in [0, CRIME_GH_N_SCHED), and of the map-stage aggregation table, in
+ * [0, 3); 0 is the default.  FFT variants 0-2 differ in tiling with radix-16 Stockham plans; variant 3
+ * is variant 0's tiling with radix-8 plans (one pass more, smaller butterflies).  All FFT variants give
+ * the same fields up to the rounding of their different butterflies (variants 0-2 bit-identical).  The
+ * map variants differ in the size of the LDS aggregation table (2048, 1024, 4096 bins: the atomic order
+ * of the FP32 sums changes).  Grids below N = 128 have one FFT variant (their entries read back as 0). */
+#define CRIME_GH_N_SCHED 4
 int crime_gh_get_schedule(const crime_gh_t *h, int *sched);
 int crime_gh_set_schedule(crime_gh_t *h, const int *sched);
 
 /* Autotuner: times every variant of every stage `reps` times on this handle's buffers, keeps the
  * fastest of each (stages in order, each tuned with the winners before it) and sets the schedule.
- * sched_out[4] (optional) receives it, ms_out[4*3] (optional) the best time of every variant (-1 when
- * the variant does not exist for this grid).  The handle then holds fields and maps, as after
- * crime_gh_run.  Batches created afterwards capture the tuned schedule. */
+ * sched_out[4] (optional) receives it, ms_out[4*CRIME_GH_N_SCHED] (optional) the best time of every
+ * variant (-1 when the variant does not exist for this grid or stage).  The handle then holds fields and
+ * maps, as after crime_gh_run.  Batches created afterwards capture the tuned schedule. */
 int crime_gh_autotune(crime_gh_t *h, int reps, int *sched_out, double *ms_out);
+
+/* The autotuner with a schedule cache (a text file, one "<key> s0 s1 s2 s3" line per tuned kind).  The
+ * key holds the device architecture, the variants' version and the handle's grid, precision, Nside,
+ * channels, samples per cell, velocity, scatter and frequency modes.  A hit sets the cached schedule
+ * without timing anything (*hit = 1); a miss runs crime_gh_autotune and appends its result (*hit = 0).
+ * hit and sched_out may be NULL. */
+int crime_gh_autotune_cached(crime_gh_t *h, const char *path, int reps, int *sched_out, int *hit);
 
 /* Device pointer and size of the resident maps (n_nu*12*Nside^2 elements of the grid precision),
  * for callers that hand the maps to further device work without a host round trip. */

@@ -1,6 +1,7 @@
-"""Kernel schedules and the autotuner: every FFT variant gives bit-identical fields, every map-stage
-variant the same maps (to the FP32 atomic order), the tuner picks the fastest timed variant of each
-stage, and batches capture the tuned schedule."""
+"""Kernel schedules and the autotuner: the tiling variants of the FFT stages give bit-identical fields and
+the radix-8 variant the same fields to FP32 rounding, every map-stage variant the same maps (to the FP32
+atomic order), the tuner picks the fastest timed variant of each stage, batches capture the tuned
+schedule, and the schedule cache returns it without timing."""
 import numpy as np
 import pytest
 
@@ -17,15 +18,22 @@ def test_fft_variants_bit_identical(cfgdir, n):
         gh.run()
         ref_d, ref_v = gh.grid(crime_amd.GRID_DENS), gh.grid(crime_amd.GRID_VPOT)
         ref_m = gh.maps(np.float64)
-        for v in range(3):
+        for v in range(4):
             for stage in range(3):
                 s = [0, 0, 0, 0]
                 s[stage] = v
                 gh.set_schedule(s)
                 assert gh.schedule() == s
                 gh.create_d_and_vr_fields()
-                assert np.array_equal(gh.grid(crime_amd.GRID_DENS), ref_d), (stage, v)
-                assert np.array_equal(gh.grid(crime_amd.GRID_VPOT), ref_v), (stage, v)
+                d, p = gh.grid(crime_amd.GRID_DENS), gh.grid(crime_amd.GRID_VPOT)
+                if v < 3:
+                    assert np.array_equal(d, ref_d), (stage, v)
+                    assert np.array_equal(p, ref_v), (stage, v)
+                else:  # radix-8 plans: other butterflies, FP32 rounding
+                    assert np.abs(d - ref_d).max() < 1e-5 * ref_d.std(), stage
+                    assert np.abs(p - ref_v).max() < 1e-5 * ref_v.std(), stage
+            if v == 3:
+                continue
             gh.set_schedule([0, 0, 0, v])
             gh.run()
             m = gh.maps(np.float64)
@@ -37,20 +45,22 @@ def test_autotune_picks_fastest_and_batch_uses_it(cfgdir):
     pars = [small_params(cfgdir, f"tune_{i}", n_grid=128, precision="single", n_side=32, l_box=1500.0, seed=50 + i)
             for i in range(2)]
     hs = [crime_amd.GetHI(p) for p in pars]
-    ref = []
-    for h in hs:
-        h.run()
-        ref.append(h.maps(np.float64))
     sched, table = hs[0].autotune(reps=2)
     print("schedule", sched, table)
     for i, st in enumerate(crime_amd.GetHI.STAGES):
         times = [t for t in table[st] if t >= 0]
-        assert len(times) == 3
+        assert len(times) == (3 if st == "pixelize" else 4)
         assert table[st][sched[i]] == min(times)
     assert hs[0].schedule() == sched
     m0 = hs[0].maps(np.float64)   # the tuner leaves the handle's maps computed
-    assert np.abs(m0 - ref[0]).max() < 2e-5 * np.abs(ref[0]).max()
+    # references with the tuned schedule (a radix-8 variant changes the fields' rounding, and through the
+    # RSD shift the channel of a few samples)
     hs[1].set_schedule(sched)
+    ref = []
+    for h in hs:
+        h.run()
+        ref.append(h.maps(np.float64))
+    assert np.abs(m0 - ref[0]).max() < 2e-5 * np.abs(ref[0]).max()
     with crime_amd.Batch(hs, n_streams=2) as b:
         b.run()
         for h, r in zip(hs, ref):
@@ -66,7 +76,28 @@ def test_small_grids_have_one_fft_variant(cfgdir):
         gh.set_schedule([2, 1, 2, 1])
         assert gh.schedule() == [0, 0, 0, 1]
         with pytest.raises(crime_amd.CrimeError):
-            gh.set_schedule([3, 0, 0, 0])
+            gh.set_schedule([4, 0, 0, 0])
+        with pytest.raises(crime_amd.CrimeError):
+            gh.set_schedule([0, 0, 0, 3])
         sched, table = gh.autotune(reps=1)
         assert sched[:3] == [0, 0, 0]
         assert table["fft_y"][1] == -1
+
+
+def test_schedule_cache(cfgdir, tmp_path):
+    """A miss tunes and appends; a hit on another handle of the same kind sets the cached schedule; another
+    kind misses."""
+    path = str(tmp_path / "schedules.txt")
+    pars = [small_params(cfgdir, f"cache_{i}", n_grid=128, precision="single", n_side=32, l_box=1500.0, seed=70 + i)
+            for i in range(2)]
+    with crime_amd.GetHI(pars[0]) as a, crime_amd.GetHI(pars[1]) as b:
+        sa, hit = a.autotune_cached(path, reps=1)
+        assert not hit
+        b.set_schedule([0, 0, 0, 0])
+        sb, hit = b.autotune_cached(path, reps=1)
+        assert hit and sb == sa and b.schedule() == sa
+    with crime_amd.GetHI(small_params(cfgdir, "cache_other", n_grid=128, n_side=16, l_box=1500.0)) as c:
+        _, hit = c.autotune_cached(path, reps=1)
+        assert not hit
+    lines = open(path).read().splitlines()
+    assert len(lines) == 2 and all(len(l.split()) == 5 for l in lines)

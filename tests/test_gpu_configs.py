@@ -1,0 +1,82 @@
+"""The two workloads of the task's configuration list that run many mocks, at their own sizes.
+
+- Batched uneven ensemble: 96 members (32 x 128^3/Nside 128, 32 x 192^3/Nside 256, 32 x 256^3/Nside 256;
+  64 channels 700-800 MHz; seeds 4000-4095; Omega_m in [0.25, 0.35], sigma_8 in [0.7, 0.9] per member)
+  in one captured graph.  With the deterministic scatter every member's maps must equal those of its own
+  single run bit for bit, on every replay; sigma^2 likewise.
+- Learned C_l emulator: trained on 512 Latin-hypercube cosmologies x 8 realisations of 256^3 / Nside 256 /
+  64-channel mocks, judged on 64 held-out cosmologies x 16 fresh realisations, against the method
+  without it (the nearest training cosmology).  Mocks and training are deterministic, so the error is
+  the same number on every run.
+"""
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import crime_amd
+import crime_amd.configs as cf
+
+pytestmark = pytest.mark.gpu
+
+
+def test_config4_ensemble_96(tmp_path):
+    nmem = 96
+    t0 = time.perf_counter()
+    inis = [cf.ensemble_member(str(tmp_path), i, extra=dict(deterministic=1)) for i in range(nmem)]
+    with ThreadPoolExecutor(16) as ex:
+        pars = list(ex.map(crime_amd.Params, inis))
+    hs = [crime_amd.GetHI(p) for p in pars]
+    print(f"ensemble: {nmem} members set up in {time.perf_counter() - t0:.1f} s", flush=True)
+    try:
+        assert sorted({(h.n, h.n_side) for h in hs}) == [(128, 128), (192, 256), (256, 256)]
+        check = [0, 13, 31, 32, 50, 63, 64, 77, 95]   # members of all three kinds
+        ref, ref_s2 = {}, {}
+        for i in check:
+            st = hs[i].run()
+            ref[i] = hs[i].maps()
+            ref_s2[i] = st["sigma2_gauss"]
+        with crime_amd.Batch(hs, n_streams=4) as b:
+            for rep in range(2):
+                for i in check:  # stale fields and maps must not survive into the replay
+                    hs[i].set_grid(crime_amd.GRID_DENS, np.zeros(hs[i].n ** 3))
+                ms = b.run()
+                print(f"ensemble replay {rep}: {ms:.1f} ms for {nmem} members", flush=True)
+                for i in check:
+                    assert np.array_equal(hs[i].maps(), ref[i]), (rep, i)
+                    assert hs[i].sigma2()[0] == ref_s2[i], (rep, i)
+        occupied = np.mean([np.count_nonzero(ref[i]) / ref[i].size for i in check])
+        assert occupied > 0.2
+    finally:
+        for h in hs:
+            h.close()
+
+
+def test_config5_emulator():
+    import crime_amd.emulator as em
+
+    spec = em.MockSpec(n_grid=256, n_side=256, n_nu=64)
+    train_c = em.latin_hypercube(512, seed=1)
+    test_c = em.latin_hypercube(64, seed=2)
+    t0 = time.perf_counter()
+    cls_tr, t_mock = em.simulate(train_c, 8, spec, seed0=100000)
+    print(f"emulator: {512 * 8} training mocks in {time.perf_counter() - t0:.1f} s", flush=True)
+    cls_te, _ = em.simulate(test_c, 16, spec, seed0=900000)
+    print(f"emulator: {64 * 16} held-out mocks done, {t_mock * 1e3:.2f} ms per mock", flush=True)
+    emu, losses = em.train_emulator(train_c, cls_tr, spec, steps=20000, batch=4096, lr0=2e-3, seed=0)
+    res = em.evaluate(emu, train_c, cls_tr, test_c, cls_te, spec)
+    xq = em.features(em.latin_hypercube(4096, seed=3), spec)
+    emu.predict(xq[:4096])
+    t1 = time.perf_counter()
+    emu.predict(xq)
+    t_eval = (time.perf_counter() - t1) / 4096
+    emu.mlp.close()
+    e, n = res["emulator"], res["nearest_cosmology"]
+    print(f"emulator: rms err/sigma {e['rms_debiased']:.3f} (median |z| {e['median_abs']:.3f}, p95 {e['p95_abs']:.3f}); "
+          f"nearest cosmology {n['rms_debiased']:.3f}; {t_eval * 1e6:.2f} us per cosmology = "
+          f"{t_mock / t_eval:.0f}x the mock rate", flush=True)
+    assert losses[-200:].mean() < 0.05 * losses[:20].mean()
+    assert e["rms_debiased"] < 0.5 * n["rms_debiased"]
+    assert e["rms_debiased"] < 0.3
+    assert t_mock / t_eval > 1000

@@ -64,3 +64,16 @@ def test_deterministic_batch_equals_single(cfgdir):
     finally:
         for h in hs:
             h.close()
+
+
+def test_set_seed_equals_fresh_handle(cfgdir):
+    """crime_gh_set_seed re-keys the mode fill: the next run equals a handle created with that seed."""
+    par = small_params(cfgdir, "det_seed", n_grid=48, n_side=16, precision="single", extra=dict(deterministic=1))
+    with crime_amd.GetHI(par, seed=11) as a, crime_amd.GetHI(par, seed=12) as b:
+        a.run()
+        first = a.maps()
+        a.set_seed(12)
+        a.run()
+        b.run()
+        assert np.array_equal(a.maps(), b.maps())
+        assert not np.array_equal(first, b.maps())

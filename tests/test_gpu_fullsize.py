@@ -136,13 +136,20 @@ def test_production_fp32_vs_fp64_run(production, tmp_path):
             total += int(np.count_nonzero(b >= 0))
     err, occ = _bin_errors(m32, m64)
     within = float(np.mean(err < 1e-5))
-    rms = float(np.sqrt(np.mean(err ** 2)))
-    print(f"1024^3 FP32 vs FP64 maps: max rel {err.max():.2e}, rms rel {rms:.2e}, {100 * within:.4f}% of "
-          f"{err.size} bins within 1e-5, occupancy mismatches {occ}; channel flips {flips} of {total} "
-          f"binned samples on 3 planes; sigma2 {st['sigma2_gauss']:.9f} vs {st64['sigma2_gauss']:.9f}")
+    med = float(np.median(err))
+    # map-level RMS: the rms of the difference over the rms of the FP64 map (per-bin relative errors are
+    # dominated by the few bins a channel flip moves a sample into or out of)
+    rms_map = float(np.sqrt(np.mean((m32 - m64) ** 2) / np.mean(m64 ** 2)))
+    print(f"1024^3 FP32 vs FP64 maps: max rel {err.max():.2e}, median rel {med:.2e}, {100 * within:.4f}% of "
+          f"{err.size} bins within 1e-5, rms(diff)/rms(map) {rms_map:.2e}, occupancy mismatches {occ}; channel "
+          f"flips {flips} of {total} binned samples on 3 planes; sigma2 {st['sigma2_gauss']:.9f} vs "
+          f"{st64['sigma2_gauss']:.9f}")
     assert abs(st["sigma2_gauss"] - st64["sigma2_gauss"]) < 1e-5 * st64["sigma2_gauss"]
-    assert within > 0.999
-    assert rms < 1e-4
+    assert within > 0.9999
+    assert med < 1e-6
+    # the rms is set by the ~1e-7 of binned samples whose channel flips with the FP32 RSD shift (the
+    # heaviest bins, near the observer, weigh most); every other bin agrees to FP32 accuracy
+    assert rms_map < 1e-3
     assert flips < 1e-4 * total
     assert abs(m32.sum() - m64.sum()) < 1e-5 * m64.sum()
 
@@ -202,21 +209,25 @@ def test_largest_2048_fp32_vs_fp64_run(largest):
         st64 = g64.run()
         m64 = g64.maps(np.float64)
     nbins = nwithin = nocc = 0
-    emax = esq = 0.0
+    emax = dsq = msq = 0.0
+    meds = []
     for c in range(m64.shape[0]):
-        err, occ = _bin_errors(m32[c].astype(np.float64), m64[c])
+        a = m32[c].astype(np.float64)
+        err, occ = _bin_errors(a, m64[c])
         nbins += err.size
         nwithin += int(np.count_nonzero(err < 1e-5))
         nocc += occ
         emax = max(emax, float(err.max()))
-        esq += float(np.sum(err ** 2))
-    rms = np.sqrt(esq / nbins)
-    print(f"2048^3 FP32 vs FP64 maps: max rel {emax:.2e}, rms rel {rms:.2e}, {100 * nwithin / nbins:.4f}% of "
-          f"{nbins} bins within 1e-5, occupancy mismatches {nocc}; sigma2 {st['sigma2_gauss']:.9f} vs "
-          f"{st64['sigma2_gauss']:.9f}")
+        meds.append(float(np.median(err)))
+        dsq += float(np.sum((a - m64[c]) ** 2))
+        msq += float(np.sum(m64[c] ** 2))
+    rms_map = np.sqrt(dsq / msq)
+    print(f"2048^3 FP32 vs FP64 maps: max rel {emax:.2e}, median rel {np.median(meds):.2e}, "
+          f"{100 * nwithin / nbins:.4f}% of {nbins} bins within 1e-5, rms(diff)/rms(map) {rms_map:.2e}, occupancy "
+          f"mismatches {nocc}; sigma2 {st['sigma2_gauss']:.9f} vs {st64['sigma2_gauss']:.9f}")
     assert abs(st["sigma2_gauss"] - st64["sigma2_gauss"]) < 1e-5 * st64["sigma2_gauss"]
-    assert nwithin > 0.999 * nbins
-    assert rms < 1e-4
+    assert nwithin > 0.9999 * nbins
+    assert rms_map < 1e-3
 
 
 def test_largest_2048_spectra(tmp_path):
