@@ -188,6 +188,8 @@ def gpu_lib():
         lib.crime_gh_screen_check.argtypes = [H, C.POINTER(C.c_int64), _dp]
         lib.crime_gh_reference_maps.restype = C.c_int
         lib.crime_gh_reference_maps.argtypes = [H, _dp]
+        lib.crime_gh_power_spectrum_noalloc.restype = C.c_int
+        lib.crime_gh_power_spectrum_noalloc.argtypes = [H, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int64)]
         lib.crime_gh_device_maps.restype = C.c_int
         lib.crime_gh_device_maps.argtypes = [H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
         lib.crime_gh_mk_psources_maps.restype = C.c_int

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -60,6 +61,19 @@ struct crime_gh {
 };
 
 static thread_local std::string g_create_err;
+
+// ---------------------------------------------------------------------------------------------
+// Scratch of crime_gh_power_spectrum_noalloc: a file-scope device array, one copy per GPU, held until
+// the process ends.  Layout (doubles): shell sums acc[3 * PK_MAX_BINS] | P[PK_MAX_BINS] |
+// k_mean[PK_MAX_BINS] | n_modes as int64[PK_MAX_BINS]: 96 KB.  In host code `pk_scratch` names a
+// host-side stand-in, not device memory: its device address is taken once per device with
+// hipGetSymbolAddress (scratch_setup, from crime_gh_create) and only that address is used.
+// ---------------------------------------------------------------------------------------------
+constexpr int PK_SCRATCH_DOUBLES = 6 * PK_MAX_BINS;
+static __device__ __align__(256) double pk_scratch[PK_SCRATCH_DOUBLES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[SCRATCH_MAX_DEVICES] = {};
 
 static int set_error(crime_gh *h, int code, const char *fmt, ...)
 {
@@ -321,6 +335,22 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return set_error(h, CRIME_GH_ENODEV, "device %d is %s; this library holds gfx950 code only", d->device,
                      prop.gcnArchName);
+
+  if (d->device >= SCRATCH_MAX_DEVICES)
+    return set_error(h, CRIME_GH_ENODEV, "device %d: at most %d devices are supported", d->device, SCRATCH_MAX_DEVICES);
+  {
+    // the device is current here: record this device's address of pk_scratch once
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[d->device]) {
+      void *p = nullptr;
+      hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(pk_scratch));
+      if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return set_error(h, CRIME_GH_EHIP, "hipGetSymbolAddress(pk_scratch): %s", hipGetErrorString(e));
+      }
+      g_scratch[d->device] = (double *)p;
+    }
+  }
 
   h->d = *d;
   h->prec = d->precision;
@@ -710,9 +740,71 @@ extern "C" int crime_gh_power_spectrum(crime_gh_t *h, int nbins, double bin_dk, 
   return 0;
 }
 
+// zero n doubles (a kernel, not a memset: no memset node if a caller captures the stream)
+__global__ __launch_bounds__(256) void k_scratch_zero(double *__restrict__ p, int n)
+{
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ p, size_t n)
 {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0;
+}
+
+// shell sums -> P, mean |k| and counts, the same operations as crime_gh_power_spectrum's host loop
+__global__ __launch_bounds__(256) void k_pk_finish(const double *__restrict__ acc, int nbins, double dk,
+                                                   double *__restrict__ pk, double *__restrict__ k_mean,
+                                                   int64_t *__restrict__ nmodes)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nbins) return;
+  const double c = acc[3 * i + 2];
+  nmodes[i] = (int64_t)c;
+  k_mean[i] = c > 0 ? acc[3 * i + 1] / c * dk : 0.0;
+  pk[i] = c > 0 ? acc[3 * i] / c : 0.0;
+}
+
+extern "C" int crime_gh_power_spectrum_noalloc(crime_gh_t *h, int nbins, double bin_dk, double *k_mean, double *pk,
+                                               int64_t *nmodes)
+{
+  if (!h) return set_error(nullptr, CRIME_GH_EINVAL, "handle is NULL");
+  if (!k_mean || !pk || !nmodes) return set_error(h, CRIME_GH_EINVAL, "NULL argument");
+  if (nbins < 1 || nbins > PK_MAX_BINS)
+    return set_error(h, CRIME_GH_EINVAL, "nbins %d not in [1, %d] (the size of the static scratch)", nbins, PK_MAX_BINS);
+  if (!(bin_dk > 0)) return set_error(h, CRIME_GH_EINVAL, "bin_dk must be > 0");
+  if (!h->fields_done) return set_error(h, CRIME_GH_ESTATE, "power_spectrum before create_d_and_vr_fields");
+  HIP_OR_FAIL(h, hipSetDevice(h->d.device));
+  double *sp;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    sp = g_scratch[h->d.device];
+  }
+  if (!sp) return set_error(h, CRIME_GH_ESTATE, "no scratch address for device %d", h->d.device);
+  double *acc = sp, *d_pk = sp + 3 * PK_MAX_BINS, *d_k = sp + 4 * PK_MAX_BINS;
+  int64_t *d_n = (int64_t *)(sp + 5 * PK_MAX_BINS);
+  const double dk = 2 * M_PI / h->d.l_box, n3 = (double)h->n * h->n * h->n;
+  const double pscale = h->d.l_box * h->d.l_box * h->d.l_box / (n3 * n3), binw = bin_dk / dk;
+  hipLaunchKernelGGL(k_scratch_zero, dim3((3 * nbins + 255) / 256), dim3(256), 0, h->stream, acc, 3 * nbins);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = h->prec ? launch_power_spectrum<double>(h->n, (cx<double> *)h->F, (const cx<double> *)h->d_tw, pscale, binw,
+                                                nbins, acc, h->stream)
+                : launch_power_spectrum<float>(h->n, (cx<float> *)h->F, (const cx<float> *)h->d_tw, pscale, binw,
+                                               nbins, acc, h->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_pk_finish, dim3((nbins + 255) / 256), dim3(256), 0, h->stream, acc, nbins, dk, d_pk, d_k, d_n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(pk, d_pk, nbins * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(k_mean, d_k, nbins * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(nmodes, d_n, nbins * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  h->fields_done = false;  // the grid now holds the transform
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(h, CRIME_GH_EHIP, "power_spectrum_noalloc: %s", hipGetErrorString(e));
+  }
+  return 0;
 }
 
 extern "C" int crime_gh_set_seed(crime_gh_t *h, uint32_t seed)

@@ -245,6 +245,15 @@ int crime_gh_screen_check(crime_gh_t *h, int64_t *counts, double *eps);
  * crime_gh_create_d_and_vr_fields must run again before crime_gh_mk_T_maps. */
 int crime_gh_power_spectrum(crime_gh_t *h, int nbins, double bin_dk, double *k_mean, double *pk, int64_t *nmodes);
 
+/* crime_gh_power_spectrum without any allocation (no counterpart in the reference).  For callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the shell sums live in a static device array of the library, one per GPU, 96 KB, set
+ * up by crime_gh_create; nbins <= 2048 (larger is CRIME_GH_EINVAL).  Same arguments, results and
+ * consumption of the fields as crime_gh_power_spectrum.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (two threads, two handles) race and must be serialised by the caller. */
+int crime_gh_power_spectrum_noalloc(crime_gh_t *h, int nbins, double bin_dk, double *k_mean, double *pk,
+                                    int64_t *nmodes);
+
 /* A new realisation on the same handle (no counterpart in the reference, which reads the seed once,
  * src/io_gh.c:239): the Philox key of the mode fill becomes `seed`; the next field stage draws the new
  * modes.  Batches capture the seed at creation.  ESTATE for handles with injected modes. */
