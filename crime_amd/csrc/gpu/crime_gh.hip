@@ -34,8 +34,7 @@ struct crime_gh {
   double *d_edges = nullptr;   // [n_nu+1]
   double *d_prefac = nullptr;  // [n_nu]
   ChanRec *d_crec = nullptr;   // channel records of the frequency buckets
-  ScreenRec *d_srec = nullptr; // FP32 screen records of the same buckets
-  float4 *d_ztab = nullptr;    // FP32 z(r) table of the screen
+  float2 *d_chwin = nullptr;   // [n_nu] decided frequency interval of each channel (FP32 screen)
   void *d_amp = nullptr;       // amplitude per |k|^2
   uint32_t *d_order = nullptr; // map-stage tiles by distance from the observer
   double *d_sub = nullptr;     // [3*n_sub]
@@ -163,71 +162,84 @@ static float f32_down(double x)
 
 // Margins and tables of the FP32 screen (MapParams::scr), from bounds on the FP32 operations (u = 2^-23;
 // v_sqrt_f32, v_rcp_f32 and v_rsq_f32 are within 1 ulp, +, *, fma and conversions within 0.5 ulp):
-//  - frequency: |p| = sqrt((float)|p|^2_FP64) within 1.5 u of |p|; the FP32 lerp of z(r) moves z by at most
-//    2 A u + 2 (1 + z) u with A = r dz/dr (the |p| error through the slope, and the rounding of the
-//    table node, fma, RSD shift and 1 + z); rcp, the product and the float NU_21 add 2.5 u relative.  So
-//    |dnu| <= nu u (2 A / (1 + z) + 4.5), taken at the worst table node of the channels' redshift range,
-//    doubled;
+//  - frequency: per cell z(r) is the parabola P through the table lerp L at r_c - h, r_c, r_c + h
+//    (h bounds the sample offsets).  On the cloud L is a line plus sum_j kappa_j (r - r_j)_+ over the
+//    nodes r_j inside it (kappa_j: the slope jumps, second differences / dr), and P reproduces lines, so
+//    L - P = sum_j kappa_j e_j with e_j the parabola's error on one hinge, |e_j| <= h / (3 sqrt 3).  With
+//    kappa_j = kb + d_j: the kb part is kb (I[q] - q - P[I[q] - q]) for the lerp I[q] of a parabola q with
+//    unit slope jumps, at most 2.25 |kb| dr / 8 (|I[q] - q| <= dr / 8, Lebesgue constant 1.25), and the d_j
+//    part at most h / (3 sqrt 3) sum |d_j|.  E_q is the largest such bound over every run of
+//    floor(2h / dr) + 1 consecutive nodes of the channels' radii, with kb the run's mid-range.  The FP32
+//    evaluation: |p| = sqrt((float)|p|^2_FP64) within 1.25 u of |p| and the rounded r_c within 0.5 u,
+//    which moves z by at most 2 A u with A = r dz/dr; the roundings of z_c + dz, a, b, the two fmas and
+//    1 + z add at most 2 (1 + z) u; rcp, the product and the float NU_21 2.5 u relative.  So
+//    |dnu| <= nu (u (2 A / (1 + z) + 4.5) + E_q / (1 + z)), taken at the worst table node of the
+//    channels' redshift range, doubled;
 //  - pixel: the angle (input rounding 0.32 u, t = min/max 0.48 u, polynomial and its evaluation 1.2 u,
 //    scalings 0.75 u) within 2.8 u quarter turns; cos(theta) within 3.3 u; sin(theta) and the cap's
 //    square root within 5 u relative; every floor argument of vec2pix is then within 8 N u of the exact
 //    one, whose FP64 evaluation is itself within 1e-12 of it: margin 16 N u.  |cos theta| and the
 //    quadrant angle w keep 8 u from 2/3 and from 0 or 1.
-static int screen_setup(crime_gh *h, const crime_gh_desc *d, const std::vector<double> &edges,
-                        const std::vector<double> &prefac, const std::vector<int> &guess, int n_guess, double g_lo,
-                        double g_inv)
+static int screen_setup(crime_gh *h, const crime_gh_desc *d, const std::vector<double> &edges)
 {
   const double u = 1.0 / 8388608.0;
   ScreenParams &c = h->map.scr;
   c = ScreenParams{};
   c.on = d->nu_mode == CRIME_GH_NU_TABLE;
-  // frequency margin
+  // bound on the sample offsets, as MapParams::h_sub
+  double hmax = 0;
+  for (int i = 0; i < d->n_sub; i++) {
+    const double *o = d->sub_off + 3 * i;
+    hmax = fmax(hmax, sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]));
+  }
+  const double hs = hmax * (1 + 1e-9) + 1e-6;
+  const double dr = 1 / d->idr;
+  // radii of the channels (with slack for the RSD shifts) and the table's derivative bounds over them
   const double nu_lo = edges[0], nu_hi = edges[d->n_nu];
   const double zlo = CRIME_GH_NU_21 / nu_hi - 1 - 0.05, zhi = CRIME_GH_NU_21 / nu_lo - 1 + 0.05;
+  const double *z = d->z_r2z;
+  int ilo = CRIME_GH_NZ, ihi = 0;
+  for (int i = 0; i < CRIME_GH_NZ; i++)
+    if (z[i] >= zlo && z[i] <= zhi) {
+      ilo = std::min(ilo, i);
+      ihi = std::max(ihi, i);
+    }
+  const int nwin = (int)floor(2 * hs / dr) + 1;  // nodes inside one cloud, at most
+  ilo = std::max(1, ilo - nwin - 2);
+  ihi = std::min(CRIME_GH_NZ - 2, ihi + nwin + 2);
+  double e_q = 0;
+  for (int j0 = ilo; j0 + nwin - 1 <= ihi; j0++) {
+    double kmin = INFINITY, kmax = -INFINITY;
+    for (int j = j0; j < j0 + nwin; j++) {
+      const double kap = (z[j + 1] - 2 * z[j] + z[j - 1]) * d->idr;
+      kmin = fmin(kmin, kap);
+      kmax = fmax(kmax, kap);
+    }
+    const double kb = 0.5 * (kmax + kmin);
+    e_q = fmax(e_q, 2.25 * fabs(kb) * dr / 8 + hs / (3 * sqrt(3.0)) * nwin * 0.5 * (kmax - kmin));
+  }
   double worst = 0;
   for (int i = 0; i + 1 < CRIME_GH_NZ; i++) {
-    const double z0 = d->z_r2z[i], z1 = d->z_r2z[i + 1];
+    const double z0 = z[i], z1 = z[i + 1];
     if (z1 < zlo || z0 > zhi) continue;
     const double A = (z1 - z0) * d->idr * d->r_r2z[i + 1];
-    const double z = z1 > 0 ? z1 : 0;
-    const double nu = CRIME_GH_NU_21 / (1 + (z0 > 0 ? z0 : 0));
-    worst = fmax(worst, nu * u * (2 * A / (1 + z) + 4.5));
+    const double za = z0 > 0 ? z0 : 0, zb = z1 > 0 ? z1 : 0;
+    const double nu = CRIME_GH_NU_21 / (1 + za);
+    worst = fmax(worst, nu * (u * (2 * A / (1 + zb) + 4.5) + e_q / (1 + za)));
   }
   const double eps_nu = 2 * fmax(worst, nu_hi * 4.5 * u);
-  // FP32 z(r) table: (r_i, z_i, slope, 0)
-  std::vector<float4> zt(CRIME_GH_NZ - 1);
-  for (int i = 0; i + 1 < CRIME_GH_NZ; i++)
-    zt[i] = make_float4((float)d->r_r2z[i], (float)d->z_r2z[i], (float)((d->z_r2z[i + 1] - d->z_r2z[i]) * d->idr), 0.f);
-  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_ztab, zt.size() * sizeof(float4)));
-  HIP_OR_FAIL(h, hipMemcpy(h->d_ztab, zt.data(), zt.size() * sizeof(float4), hipMemcpyHostToDevice));
-  // screen records: [edge + eps, next edge - eps), rounded inwards
-  std::vector<ScreenRec> sr(n_guess);
-  for (int j = 0; j < n_guess; j++) {
-    const int ch = guess[j], last = ch == d->n_nu - 1;
-    const double e0 = edges[ch], e1 = edges[ch + 1], e2 = last ? e1 : edges[ch + 2];
-    ScreenRec r{};
-    r.c = ch;
-    r.lo0 = f32_up(e0 + eps_nu);
-    r.hi0 = f32_down(e1 - eps_nu);
-    r.lo1 = f32_up(e1 + eps_nu);
-    r.hi1 = f32_down(e2 - eps_nu);
-    if (last) {
-      r.lo1 = INFINITY;
-      r.hi1 = -INFINITY;
-    }
-    r.pf0 = prefac[ch];
-    r.pf1 = last ? 0.0 : prefac[ch + 1];
-    sr[j] = r;
-  }
-  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_srec, sr.size() * sizeof(ScreenRec)));
-  HIP_OR_FAIL(h, hipMemcpy(h->d_srec, sr.data(), sr.size() * sizeof(ScreenRec), hipMemcpyHostToDevice));
-  c.ztab = h->d_ztab;
-  c.srec = h->d_srec;
-  c.idr = (float)d->idr;
+  // decided intervals [edge + eps, next edge - eps), rounded inwards
+  std::vector<float2> w(d->n_nu);
+  for (int k = 0; k < d->n_nu; k++) w[k] = make_float2(f32_up(edges[k] + eps_nu), f32_down(edges[k + 1] - eps_nu));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_chwin, w.size() * sizeof(float2)));
+  HIP_OR_FAIL(h, hipMemcpy(h->d_chwin, w.data(), w.size() * sizeof(float2), hipMemcpyHostToDevice));
+  c.chwin = h->d_chwin;
+  // the cloud [r_c - h, r_c + h] stays inside (0, r_last): no clamp of the table lerp inside it
+  c.rc_lo = hs * (1 + 1e-9) + 1e-6;
+  c.rc_hi = d->r_r2z[CRIME_GH_NZ - 1] - hs * (1 + 1e-9) - 1e-6;
+  c.inv_2h = 1 / (2 * hs);
+  c.inv_2h2 = 1 / (2 * hs * hs);
   c.r_min = (float)(1e-3 * d->l_box / d->n_grid);
-  c.z_last = (float)d->z_r2z[CRIME_GH_NZ - 1];
-  c.g_lo = (float)g_lo;
-  c.g_inv = (float)g_inv;
   c.out_lo = f32_down(nu_lo - eps_nu);
   c.out_hi = f32_up(nu_hi + eps_nu);
   c.eps_pix = (float)(16 * (double)d->n_side * u);
@@ -297,7 +309,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_srec, h->d_ztab, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub,
+  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_chwin, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig, h->d_ps, h->d_ps_count, h->ps_maps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -462,7 +474,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_crec, n_guess * sizeof(ChanRec)));
   HIP_OR_FAIL(h, hipMemcpy(h->d_crec, crec.data(), n_guess * sizeof(ChanRec), hipMemcpyHostToDevice));
   {
-    const int rc = screen_setup(h, d, edges, prefac, guess, n_guess, g_lo, g_inv);
+    const int rc = screen_setup(h, d, edges);
     if (rc) return rc;
   }
 
@@ -512,6 +524,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   m.n = h->n;
   m.dx = d->l_box / h->n;
   for (int i = 0; i < 3; i++) m.obs[i] = d->pos_obs[i];
+  m.inv_dx = 1.0 / m.dx;
   m.tab.z = h->d_tab;
   m.tab.r = h->d_tab + CRIME_GH_NZ;
   m.tab.d = h->d_tab + 2 * CRIME_GH_NZ;
@@ -574,7 +587,12 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     std::vector<uint32_t> count(nbins + 1, 0), order(ntiles);
     for (size_t b = 0; b < ntiles; b++) count[std::min(dist_bin(b), nbins - 1) + 1]++;
     for (size_t i = 1; i <= nbins; i++) count[i] += count[i - 1];
-    for (size_t b = 0; b < ntiles; b++) order[count[std::min(dist_bin(b), nbins - 1)]++] = (uint32_t)b;
+    // each entry holds the tile's coordinates, tx | ty << 10 | tz << 20 (ntx, nty, ntz <= 512 for n <= 2048)
+    for (size_t b = 0; b < ntiles; b++) {
+      const uint32_t tx = (uint32_t)(b % m.ntx), ty = (uint32_t)((b / m.ntx) % m.nty),
+                     tz = (uint32_t)(b / ((size_t)m.ntx * m.nty));
+      order[count[std::min(dist_bin(b), nbins - 1)]++] = tx | ty << 10 | tz << 20;
+    }
     HIP_OR_FAIL(h, hipMalloc((void **)&h->d_order, ntiles * sizeof(uint32_t)));
     HIP_OR_FAIL(h, hipMemcpy(h->d_order, order.data(), ntiles * sizeof(uint32_t), hipMemcpyHostToDevice));
     m.tile_order = h->d_order;

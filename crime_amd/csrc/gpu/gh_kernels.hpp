@@ -34,11 +34,12 @@ constexpr int emax() { return sizeof(T) == 4 ? 64 : 32; }
 
 // Schedule variants the autotuner chooses between, per stage (variant 0 is the default).  FFT stages:
 // variants 0-2 differ in tiling with radix-16 plans, variant 3 is variant 0's tiling with radix-8 plans
-// (one pass more, smaller butterflies).  The map stage has N_SCHED_PIX variants.  Grids below
-// TUNE_MIN_N have variant 0 only for the FFT stages.
+// (one pass more, smaller butterflies).  Map stage: 2048 LDS bins grouped by 4 adjacent map words
+// (variant 0) or by 8 (variant 1), 4096 bins (2) and 1024 bins (3).  Grids below TUNE_MIN_N have variant
+// 0 only for the FFT stages.
 constexpr int N_SCHED = 4;
-constexpr int N_SCHED_PIX = 3;
-constexpr int CRIME_GH_SCHED_VERSION = 2;  // bump when a variant's meaning changes (schedule cache keys)
+constexpr int N_SCHED_PIX = 4;
+constexpr int CRIME_GH_SCHED_VERSION = 4;  // bump when a variant's meaning changes (schedule cache keys)
 constexpr int TUNE_MIN_N = 128;
 struct Schedule {
   int fill = 0, y = 0, x = 0, pix = 0;
@@ -135,22 +136,13 @@ hipError_t launch_amp_table(const FillParams &f, void *tab, hipStream_t s);
 // FP32 screen of the per-sample decisions (channel and RING pixel).  Each decision is taken from FP32
 // arithmetic when its argument lies farther than a certified margin from the decision boundary, and
 // from the FP64 path (the reference's arithmetic) otherwise; both give the same integers.
-// Screen record of a frequency bucket: the channel c is decided when lo0 <= nu < hi0, channel c + 1
-// when lo1 <= nu < hi1; the bounds are the channel edges moved inwards by the frequency margin and
-// rounded inwards to float.
-struct __align__(16) ScreenRec {
-  float lo0, hi0, lo1, hi1;
-  double pf0, pf1;  // mK factors of channels c, c + 1 (the FP64 record's values)
-  int c, pad0, pad1, pad2;
-};
-
 struct ScreenParams {
   int on;                  // 0: every decision in FP64
-  const float4 *ztab;      // [NZ - 1]: (r_i, z_i, (z_{i+1} - z_i) idr, 0)
-  const ScreenRec *srec;   // [n_guess]
-  float idr, r_min;        // table scale; radii below r_min are left to FP64
-  float z_last;            // z of the last table node (z(r) beyond it)
-  float g_lo, g_inv;       // buckets in FP32 (the record check is exact whatever bucket is read)
+  const float2 *chwin;     // [n_nu] decided interval of each channel: its edges moved inwards by the
+                           // frequency margin, rounded inwards to float
+  double rc_lo, rc_hi;     // cells screened: rc_lo < r_c < rc_hi (the cloud keeps off r = 0 and the table end)
+  double inv_2h, inv_2h2;  // 1 / (2 h_sub), 1 / (2 h_sub^2): the per-cell parabola of z(r)
+  float r_min;             // radii below r_min are left to FP64
   float out_lo, out_hi;    // nu < out_lo or nu >= out_hi: no channel (nu_lo - eps, nu_hi + eps)
   float eps_pix;           // margin of the pixel floors, in the units of their arguments
   float eps_ax;            // margin of the quadrant / octant boundaries (w near 0 or 1), quarter turns
@@ -160,6 +152,7 @@ struct ScreenParams {
 struct MapParams {
   int n;
   double dx, obs[3];
+  double inv_dx;           // 1 / dx (the same double as the division)
   RTables tab;
   int bias_model;
   double b0, b1;
@@ -178,7 +171,8 @@ struct MapParams {
   const ChanRec *crec;     // [n_guess] channel records of the buckets of [g_lo, g_lo + n_guess/g_inv) (table mode)
   int n_guess;
   double g_lo, g_inv;
-  const uint32_t *tile_order;  // tiles by increasing distance of their centre from the observer
+  const uint32_t *tile_order;  // tiles by increasing distance of their centre from the observer, as
+                               // tx | ty << 10 | tz << 20
   int ntx, nty, ntz;
   int vmode;               // CRIME_GH_VEL_*: SPECTRAL reads vx from F.y and (vy, vz) from G
   const void *G;           // cx<T>[N^3] second grid of SPECTRAL handles (NULL otherwise)

@@ -30,6 +30,7 @@ struct CellState {
   double vr;           // radial velocity (fourier.c:368)
   double dz;           // RSD shift (grid_tools.c:142)
   double mass;         // HI mass (grid_tools.c:144)
+  double zc;           // z_of_r(r) (grid_tools.c:138)
 };
 
 template <typename T>
@@ -68,7 +69,7 @@ __device__ __forceinline__ void cell_velocity(const MapParams &m, const cx<T> *_
     double px1 = (double)F[row + ixp].y, px0 = (double)F[row + ixm].y;
     double py1 = (double)F[ix + nn * iyp + n2 * iz].y, py0 = (double)F[ix + nn * iym + n2 * iz].y;
     double pz1 = (double)F[ix + nn * iy + n2 * izp].y, pz0 = (double)F[ix + nn * iy + n2 * izm].y;
-    double idx = 1. / m.dx;
+    double idx = m.inv_dx;
     v0 = 0.5 * idx * (px1 - px0);
     v1 = 0.5 * idx * (py1 - py0);
     v2 = 0.5 * idx * (pz1 - pz0);
@@ -80,13 +81,34 @@ __device__ __forceinline__ void cell_velocity(const MapParams &m, const cx<T> *_
 }
 
 // lognormal HI mass and RSD shift (grid_tools.c:138-146)
-__device__ __forceinline__ void cell_hi(const MapParams &m, double delta, double sigma2, CellState &s)
+// the radial tables at the cell centre, split from the arithmetic so that a kernel can issue their loads
+// with the grid's: z, D and V of r
+struct CellTabs {
+  double z, d, v;
+};
+__device__ __forceinline__ void cell_tabs(const MapParams &m, const CellState &s, CellTabs &t)
 {
-  double redshift = z_of_r(m.tab, s.r);
-  double gfd = dgrowth_of_r(m.tab, s.r) * bias_of(m, redshift);
-  s.dz = m.rsd ? s.vr * vgrowth_of_r(m.tab, s.r) : 0.0;
+  t.z = z_of_r(m.tab, s.r);
+  t.d = dgrowth_of_r(m.tab, s.r);
+  t.v = vgrowth_of_r(m.tab, s.r);
+}
+
+__device__ __forceinline__ void cell_hi_tabs(const MapParams &m, double delta, double sigma2, const CellTabs &t,
+                                             CellState &s)
+{
+  const double redshift = t.z;
+  s.zc = redshift;
+  double gfd = t.d * bias_of(m, redshift);
+  s.dz = m.rsd ? s.vr * t.v : 0.0;
   double dens_ln = exp(gfd * (delta - 0.5 * gfd * sigma2));
   s.mass = m.mass_prefac * fraction_hi(redshift) * dens_ln;
+}
+
+__device__ __forceinline__ void cell_hi(const MapParams &m, double delta, double sigma2, CellState &s)
+{
+  CellTabs t;
+  cell_tabs(m, s, t);
+  cell_hi_tabs(m, delta, sigma2, t, s);
 }
 
 // The sample offsets are the same for every lane and never written by a kernel: read through the
@@ -104,17 +126,25 @@ struct SamplePre {
   ChanRec rec;  // table mode only
 };
 
+// from the sample's position (x, y, z) and the cell's RSD shift dz
+template <bool TABLE>
+__device__ __forceinline__ void sample_pre_pos(const MapParams &m, double x, double y, double z, double dz,
+                                               SamplePre &q)
+{
+  q.x = x;
+  q.y = y;
+  q.z = z;
+  q.r = sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+  const double redshift = z_of_r(m.tab, q.r) + dz;
+  q.nu = 1420.40575177 / (1 + redshift);
+  if constexpr (TABLE) q.rec = m.crec[channel_bucket(q.nu, m.n_guess, m.g_lo, m.g_inv)];
+}
+
 template <bool TABLE>
 __device__ __forceinline__ void sample_pre(const MapParams &m, const CellState &s, int isub, SamplePre &q)
 {
   const const_double *so = (const const_double *)m.sub_off;
-  q.x = s.x0 + so[3 * isub];
-  q.y = s.y0 + so[3 * isub + 1];
-  q.z = s.z0 + so[3 * isub + 2];
-  q.r = sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
-  const double redshift = z_of_r(m.tab, q.r) + s.dz;
-  q.nu = 1420.40575177 / (1 + redshift);
-  if constexpr (TABLE) q.rec = m.crec[channel_bucket(q.nu, m.n_guess, m.g_lo, m.g_inv)];
+  sample_pre_pos<TABLE>(m, s.x0 + so[3 * isub], s.y0 + so[3 * isub + 1], s.z0 + so[3 * isub + 2], s.dz, q);
 }
 
 template <bool TABLE>
@@ -155,13 +185,83 @@ __device__ __forceinline__ int sample_bin(const MapParams &m, const CellState &s
 // exceed the difference between the FP32 argument and the exact one, and the FP64 path's own
 // difference from the exact one is far below them, so an accepted decision is the FP64 path's.  Any
 // decision that misses its margin is taken by the FP64 path of sample_pre/sample_post/vec2pix_ring.
+//
+// The channel needs no memory access per sample.  Per cell: z(r) over the sample cloud (|r - r_c| <= h)
+// is the parabola through the FP64 table lerp at r_c - h, r_c and r_c + h, whose distance from the lerp
+// the host bounds and adds to the margin; and the decided frequency intervals [lo, hi) of the three
+// channels around the centre's channel (CellScreen).  Per sample: the parabola, 1 + z, a reciprocal
+// and two comparisons give the window slot k of the channel.
 // ---------------------------------------------------------------------------------------------
 struct SampleS {
-  double x, y, z;         // position relative to the observer (the reference's FP64 sums)
-  float xf, yf, zf, rf;   // the same in FP32, and |p| from them
+  float xf, yf, zf, rf;   // position relative to the observer (the reference's FP64 sums) in FP32, and |p|
   int inu;                // channel, -1 none, -2 undecided
-  double pf;              // mK factor of the channel
+  int k;                  // window slot of the channel (0..2)
 };
+
+struct CellScreen {
+  float rc, zcd, a, b;                 // z(r_c + t) + dz = zcd + t (a + b t); zcd NaN: no screen for the cell
+  float lo0, hi0, lo1, hi1, lo2, hi2;  // decided intervals of channels c0, c0 + 1, c0 + 2 (empty outside),
+                                       // disjoint and in increasing order
+  int c0;
+};
+
+// the FP64 position of sample isub of a cell (the reference's sums; recomputed where the FP64 path needs it)
+__device__ __forceinline__ void sample_pos(const MapParams &m, const CellState &s, int isub, double &x, double &y,
+                                           double &z)
+{
+  const const_double *so = (const const_double *)m.sub_off;
+  x = s.x0 + so[3 * isub];
+  y = s.y0 + so[3 * isub + 1];
+  z = s.z0 + so[3 * isub + 2];
+}
+
+// decided interval of channel c: empty below channel 0 (-inf, -inf) and above the last (+inf, +inf), so
+// that the lower ends stay ordered across the window
+__device__ __forceinline__ float2 chan_window(const MapParams &m, int c)
+{
+  if (c < 0) return make_float2(-INFINITY, -INFINITY);
+  if (c >= m.n_nu) return make_float2(INFINITY, INFINITY);
+  return m.scr.chwin[c];
+}
+
+// false when no sample of the cell can land in a channel (conservative); zm, zp as cell_z_range
+__device__ __forceinline__ bool cell_may_bin(const MapParams &m, const CellState &s, double zm, double zp)
+{
+  double zlo = zm + s.dz;
+  double zhi = zp + s.dz;
+  if (!(1 + zlo > 0) || !(1 + zhi > 0)) return true;
+  // nu_top = NU_21 / (1 + zlo) below nu_lo or nu_bot = NU_21 / (1 + zhi) at or above nu_hi, multiplied out;
+  // the 1e-10 slack covers every rounding of these and of the samples' frequencies
+  if (1420.40575177 * (1 + 1e-10) < m.nu_lo * (1 + zlo)) return false;
+  if (1420.40575177 * (1 - 1e-10) >= m.nu_hi * (1 + zhi)) return false;
+  return true;
+}
+
+// per-cell part of the screen; zm, zp = z_of_r(r_c -+ h_sub) and s.zc = z_of_r(r_c) (TABLE mode)
+__device__ __forceinline__ void screen_cell(const MapParams &m, const CellState &s, double zm, double zp,
+                                            CellScreen &cs)
+{
+  const ScreenParams &c = m.scr;
+  const bool ok = s.r > c.rc_lo && s.r < c.rc_hi;  // the cloud keeps off r = 0 and the table's end
+  cs.rc = (float)s.r;
+  // a cell none of whose samples can reach a channel: z = inf gives nu = 0, no channel for every sample
+  // (the FP64 path's decision); otherwise NaN leaves every sample of an unscreened cell to FP64
+  cs.zcd = !cell_may_bin(m, s, zm, zp) ? INFINITY : (ok ? (float)(s.zc + s.dz) : __builtin_nanf(""));
+  cs.a = (float)((zp - zm) * c.inv_2h);
+  cs.b = (float)((zp - 2.0 * s.zc + zm) * c.inv_2h2);
+  // the channel of the centre's frequency, from its bucket record; the window is that channel and its
+  // neighbours (any window is correct: frequencies outside it are left to FP64)
+  const double nu_c = 1420.40575177f * __builtin_amdgcn_rcpf((float)(1 + (s.zc + s.dz)));
+  const ChanRec &r = m.crec[channel_bucket(nu_c, m.n_guess, m.g_lo, m.g_inv)];
+  cs.c0 = (nu_c < r.e1 ? r.c : r.c + 1) - 1;
+  const float2 w0 = chan_window(m, cs.c0), w1 = chan_window(m, cs.c0 + 1), w2 = chan_window(m, cs.c0 + 2);
+  cs.lo0 = w0.x;
+  cs.hi0 = w0.y;
+  cs.lo1 = w1.x;
+  cs.hi1 = w1.y;
+  cs.lo2 = w2.x;
+  cs.hi2 = w2.y;
+}
 
 // atan(t) on [0, 1] as t P(t^2), P of degree 7 fitted to atan in absolute error; evaluated with FMAs the
 // result is within 1.4e-7 of atan(t) for every float t in [0, 1]
@@ -179,44 +279,35 @@ __device__ __forceinline__ float atan01(float t)
   return p * t;
 }
 
-// position, |p| and the channel from FP32 (TABLE mode): q.inu = channel, -1 (no channel) or -2 (left to FP64)
-__device__ __forceinline__ void screen_channel(const MapParams &m, const CellState &s, float dzf, int isub, SampleS &q)
+// position, |p| and the channel from FP32 (TABLE mode): q.inu = channel, -1 (no channel) or -2 (left to
+// FP64), q.k its window slot
+__device__ __forceinline__ void screen_channel(const MapParams &m, const CellState &s, const CellScreen &cs, int isub,
+                                               SampleS &q)
 {
   const ScreenParams &c = m.scr;
-  const const_double *so = (const const_double *)m.sub_off;
-  q.x = s.x0 + so[3 * isub];
-  q.y = s.y0 + so[3 * isub + 1];
-  q.z = s.z0 + so[3 * isub + 2];
-  q.xf = (float)q.x;
-  q.yf = (float)q.y;
-  q.zf = (float)q.z;
+  double x, y, z;
+  sample_pos(m, s, isub, x, y, z);
+  q.xf = (float)x;
+  q.yf = (float)y;
+  q.zf = (float)z;
   // |p|^2 in FP64 (exact to 1e-16), its root in FP32: |p| within 1.25 ulp
-  q.rf = __builtin_amdgcn_sqrtf((float)(q.x * q.x + q.y * q.y + q.z * q.z));
-  const int ir = (int)fminf(fmaxf(q.rf * c.idr, 0.0f), (float)(5001 - 2));
-  const float4 e = c.ztab[ir];
-  // past the last node z(r) is the last entry, as in tab_lerp (the lerp is continuous there)
-  const float zr = fminf(fmaf(e.z, q.rf - e.x, e.y), c.z_last) + dzf;
-  const float nu = 1420.40575177f * __builtin_amdgcn_rcpf(1.0f + zr);
-  const int b = (int)fminf(fmaxf((nu - c.g_lo) * c.g_inv, 0.0f), (float)(m.n_guess - 1));
-  const ScreenRec r = c.srec[b];
-  int inu = -2;
-  if (nu >= r.lo0 && nu < r.hi0) {
-    inu = r.c;
-    q.pf = r.pf0;
-  }
-  else if (nu >= r.lo1 && nu < r.hi1) {
-    inu = r.c + 1;
-    q.pf = r.pf1;
-  }
-  else if (nu < c.out_lo || nu >= c.out_hi) {
-    inu = -1;
-  }
+  q.rf = __builtin_amdgcn_sqrtf((float)(x * x + y * y + z * z));
+  const float t = q.rf - cs.rc;
+  const float zs = fmaf(t, fmaf(t, cs.b, cs.a), cs.zcd);
+  const float nu = 1420.40575177f * __builtin_amdgcn_rcpf(1.0f + zs);
+  // three interval tests (a select among the struct's fields by k would leave the struct in memory)
+  const bool d0 = nu >= cs.lo0 && nu < cs.hi0;
+  const bool d1 = nu >= cs.lo1 && nu < cs.hi1;
+  const bool d2 = nu >= cs.lo2 && nu < cs.hi2;
+  const int k = d1 ? 1 : (d2 ? 2 : 0);
+  int inu = (d0 || d1 || d2) ? cs.c0 + k : ((nu < c.out_lo || nu >= c.out_hi) ? -1 : -2);
   if (!(q.rf > c.r_min)) inu = -2;  // near the observer
   q.inu = inu;
+  q.k = k;
 }
 
-// RING pixel from FP32, or -2 when a margin is not met (see vec2pix_ring for the FP64 form)
-__device__ __forceinline__ int64_t screen_pixel(const MapParams &m, const SampleS &q)
+// RING pixel from FP32, or -2 when a margin is not met (see vec2pix_ring for the FP64 form); npix < 2^31
+__device__ __forceinline__ int screen_pixel(const MapParams &m, const SampleS &q)
 {
   const ScreenParams &c = m.scr;
   const int nside = (int)m.nside;
@@ -249,7 +340,7 @@ __device__ __forceinline__ int64_t screen_pixel(const MapParams &m, const Sample
     const int kshift = 1 - (ir & 1);
     int ip = (jp + jm - nside + kshift + 1) / 2;
     ip = imod_small(ip, 4 * nside);
-    return (int64_t)(nside * (nside - 1) * 2 + (ir - 1) * 4 * nside + ip);
+    return nside * (nside - 1) * 2 + (ir - 1) * 4 * nside + ip;
   }
   // polar caps: tp = tt - (int)tt, jp = floor(tp tmp), jm = floor((1 - tp) tmp), tmp = N sin(theta) sqrt(3/(1+za))
   const float tp = sw > 0.0f ? w : 1.0f - w;
@@ -267,57 +358,67 @@ __device__ __forceinline__ int64_t screen_pixel(const MapParams &m, const Sample
   if (d3 < eps || d3 > 1.0f - eps) return -2;
   int ip = B * ir + (int)g3;
   ip = imod_small(ip, 4 * ir);
-  if (cth > 0.0f) return (int64_t)(2 * ir * (ir - 1) + ip);
-  return (int64_t)(12 * nside * nside - 2 * ir * (ir + 1) + ip);
+  if (cth > 0.0f) return 2 * ir * (ir - 1) + ip;
+  return 12 * nside * nside - 2 * ir * (ir + 1) + ip;
 }
 
-// finish a screened sample: undecided channels take the whole FP64 path, undecided pixels the FP64
-// vec2pix_ring; returns the channel (-1: not binned) with its pixel in ipix and mK factor in q.pf
+// finish a screened sample in its step: undecided channels take the whole FP64 path, undecided pixels
+// the FP64 vec2pix_ring; returns the channel (-1: not binned) with its pixel in ipix
 __device__ __forceinline__ int resolve_sample(const MapParams &m, const CellState &s, int isub, SampleS &q,
                                               int64_t &ipix)
 {
   if (q.inu == -2) {
     SamplePre p;
+    double pf;
     sample_pre<true>(m, s, isub, p);
-    q.inu = sample_post<true>(m, p, ipix, q.pf);
+    q.inu = sample_post<true>(m, p, ipix, pf);
     return q.inu;
   }
   ipix = -1;
   if (q.inu < 0) return -1;
   ipix = screen_pixel(m, q);
-  if (ipix == -2) ipix = vec2pix_ring(m.nside, q.x, q.y, q.z, sqrt(q.x * q.x + q.y * q.y + q.z * q.z));
+  if (ipix == -2) {
+    double x, y, z;
+    sample_pos(m, s, isub, x, y, z);
+    ipix = vec2pix_ring(m.nside, x, y, z, sqrt(x * x + y * y + z * z));
+  }
   return q.inu;
 }
 
-// the channel and pixel every map kernel uses: screened in TABLE mode when the screen is on
-__device__ __forceinline__ int sample_decide(const MapParams &m, const CellState &s, float dzf, int isub, int64_t &ipix,
-                                             double &pf)
+// z of the cloud's extreme radii (the screen's parabola and cell_may_bin)
+__device__ __forceinline__ void cell_z_range(const MapParams &m, const CellState &s, double &zm, double &zp)
+{
+  zm = z_of_r(m.tab, s.r - m.h_sub);
+  zp = z_of_r(m.tab, s.r + m.h_sub);
+}
+
+// Per-cell state of sample_decide: the screen's when it is on (TABLE mode)
+__device__ __forceinline__ void decide_cell(const MapParams &m, const CellState &s, CellScreen &cs)
+{
+  if (m.nu_mode == 0 && m.scr.on) {
+    double zm, zp;
+    cell_z_range(m, s, zm, zp);
+    screen_cell(m, s, zm, zp, cs);
+  }
+}
+
+// the channel and pixel every map kernel uses: screened in TABLE mode when the screen is on; pf is the
+// channel's mK factor (prefac[inu], the value the FP64 path reads from the channel record)
+__device__ __forceinline__ int sample_decide(const MapParams &m, const CellState &s, const CellScreen &cs, int isub,
+                                             int64_t &ipix, double &pf)
 {
   if (m.nu_mode == 0 && m.scr.on) {
     SampleS q;
-    screen_channel(m, s, dzf, isub, q);
+    screen_channel(m, s, cs, isub, q);
     const int inu = resolve_sample(m, s, isub, q, ipix);
-    pf = q.pf;
+    pf = inu >= 0 ? m.prefac[inu] : 0.0;
     return inu;
   }
   return sample_bin(m, s, isub, ipix, pf);
 }
 
-// false when no sample of the cell can land in a channel (conservative)
-__device__ __forceinline__ bool cell_may_bin(const MapParams &m, const CellState &s)
-{
-  double zlo = z_of_r(m.tab, s.r - m.h_sub) + s.dz;
-  double zhi = z_of_r(m.tab, s.r + m.h_sub) + s.dz;
-  if (!(1 + zlo > 0) || !(1 + zhi > 0)) return true;
-  double nu_top = 1420.40575177 / (1 + zlo);
-  double nu_bot = 1420.40575177 / (1 + zhi);
-  if (nu_top * (1 + 1e-10) < m.nu_lo) return false;
-  if (nu_bot * (1 - 1e-10) >= m.nu_hi) return false;
-  return true;
-}
-
-// Add val to bin `key` of the workgroup's LDS table (open addressing, linear probing); a sample that
-// finds no slot within the probe budget goes straight to the map in HBM.
+// Add val to bin `key` of the workgroup's LDS table (open addressing, linear probing from slot h); a sample
+// that finds no slot within the probe budget goes straight to the map in HBM.
 template <typename T>
 __device__ __forceinline__ void global_add(T *p, T v)
 {
@@ -325,24 +426,31 @@ __device__ __forceinline__ void global_add(T *p, T v)
   else unsafeAtomicAdd(p, v);
 }
 
-template <typename T, int SLOTS>
-__device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned long long *keys, T *vals,
-                                        T *__restrict__ maps)
+// Home slot of a bin: the 2^SEG keys of one aligned run of map addresses (one 2^SEG-word segment) start at
+// a slot given by 32-bit Fibonacci hashing of the segment and follow each other from there, so that the
+// flush, where the lanes of a wave read consecutive slots, hands the lanes of one segment to one atomic
+// instruction: the memory system merges the lanes of an instruction that fall into one segment (runs of
+// 4 lanes per 32-byte segment cost a third of scattered lanes).
+template <int SLOTS, int SEG, typename K>
+__device__ __forceinline__ unsigned home_slot(K key)
 {
-  const unsigned long long EMPTY = ~0ull;
-  // 32-bit Fibonacci hashing of the folded key (one v_mul_lo_u32 instead of a 64-bit product)
   constexpr int SHIFT = 32 - __builtin_ctz(SLOTS);
-  unsigned h = (((unsigned)key ^ (unsigned)(key >> 32)) * 0x9E3779B1u) >> SHIFT;
+  const K seg = key >> SEG;
+  unsigned folded;
+  if constexpr (sizeof(K) == 8) folded = (unsigned)seg ^ (unsigned)((unsigned long long)seg >> 32);
+  else folded = (unsigned)seg;
+  return (((folded * 0x9E3779B1u) >> SHIFT) + ((unsigned)key & ((1u << SEG) - 1))) & (SLOTS - 1);
+}
+
+template <typename T, int SLOTS, typename K>
+__device__ __forceinline__ void agg_add(K key, unsigned h, T val, K *keys, T *vals, T *__restrict__ maps)
+{
+  const K EMPTY = (K)~(K)0;
+  // the compare-and-swap claims an empty slot and returns the key of an occupied one: one LDS round trip
+  // per probe
   for (int p = 0; p < 64; p++) {
-    unsigned long long k = keys[h];
-    if (k == EMPTY) {
-      unsigned long long old = atomicCAS(&keys[h], EMPTY, key);
-      if (old == EMPTY || old == key) {
-        atomicAdd(&vals[h], val);
-        return;
-      }
-    }
-    else if (k == key) {
+    const K old = atomicCAS(&keys[h], EMPTY, key);
+    if (old == EMPTY || old == key) {
       atomicAdd(&vals[h], val);
       return;
     }
@@ -351,19 +459,54 @@ __device__ __forceinline__ void agg_add(unsigned long long key, T val, unsigned 
   global_add(maps + key, val);
 }
 
+// Samples whose FP32 decision missed a margin, queued per wave in LDS and resolved by the FP64 path
+// together by all lanes of the wave after the last sample, so that the FP64 path runs about once per
+// wave instead of in every sample step in which one lane needs it (with 128 samples per wave and step,
+// that is most steps).  A sample that finds the queue full is resolved in its step.  dz is the cell's
+// RSD shift, w its mass per sample, inu the channel or -2 when the channel is undecided.
+constexpr int DEFER_CAP = 32;  // entries per wave
+struct DeferQueue {
+  double x[4][DEFER_CAP], y[4][DEFER_CAP], z[4][DEFER_CAP], dz[4][DEFER_CAP], w[4][DEFER_CAP];
+  int inu[4][DEFER_CAP];
+};
+
+// rank of this lane among the lanes set in mask below it
+__device__ __forceinline__ int lane_rank(unsigned long long mask)
+{
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// the FP64 decision of a deferred sample: the whole sample_pre/sample_post path when the channel is
+// undecided, else the pixel only (the same operations as the unscreened path); the channel's factor
+// prefac[inu] is the one its screen record holds
+__device__ __forceinline__ int resolve_deferred(const MapParams &m, double x, double y, double z, double dz, int inu,
+                                                int64_t &ipix, double &pf)
+{
+  if (inu == -2) {
+    SamplePre p;
+    sample_pre_pos<true>(m, x, y, z, dz, p);
+    return sample_post<true>(m, p, ipix, pf);
+  }
+  pf = m.prefac[inu];
+  ipix = vec2pix_ring(m.nside, x, y, z, sqrt(x * x + y * y + z * z));
+  return inu;
+}
+
 // One workgroup = one TILE_X x TILE_Y x TILE_Z block of cells (one cell per thread).  Samples are
 // summed per (channel, pixel) bin in LDS and every occupied bin is added to HBM once, so the
-// global atomics scale with the bins a tile touches instead of with its samples.  SLOTS (the
-// schedule variant: 2048, 1024 or 4096 bins) trades occupancy against probe length and overflow.
+// global atomics scale with the bins a tile touches instead of with its samples.  SLOTS (2048, 1024 or
+// 4096 bins) trades occupancy against probe length and overflow; SEG groups the bins of 2^SEG adjacent
+// map words (home_slot); K is the bin key type (32-bit when n_nu npix < 2^32 - 1, which halves the key
+// table).  In TABLE mode the FP32 screen decides the samples and its FP64 fallbacks are queued per wave.
 // FX (deterministic scatter): every sample adds the integer round(s mass / n_sub) to the 64-bit bin of
 // m.fxmaps instead of its FP mK value to maps; k_fx_finalize applies 1/s and the channel factor.
-template <typename T, int SLOTS, bool FX = false>
-__global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T *__restrict__ maps,
-                                                  MapParams m)
+template <typename T, int SLOTS, int SEG, bool FX, typename K>
+__device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__restrict__ maps, const MapParams &m)
 {
   using V = typename std::conditional<FX, unsigned long long, T>::type;
-  __shared__ unsigned long long keys[SLOTS];
+  __shared__ K keys[SLOTS];
   __shared__ V vals[SLOTS];
+  __shared__ DeferQueue dq;
   V *__restrict__ vmaps;
   if constexpr (FX) vmaps = m.fxmaps;
   else vmaps = maps;
@@ -374,80 +517,94 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
   // most 2^32 work-items per dimension.
   const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
   if (bid >= (size_t)m.ntx * m.nty * m.ntz) return;
-  const size_t b = m.tile_order[bid];
-  const int tx = (int)(b % m.ntx), ty = (int)((b / m.ntx) % m.nty), tz = (int)(b / ((size_t)m.ntx * m.nty));
+  const uint32_t b = m.tile_order[bid];
+  const int tx = (int)(b & 1023u), ty = (int)((b >> 10) & 1023u), tz = (int)(b >> 20);
   const int ix = tx * TILE_X + (t % TILE_X), iy = ty * TILE_Y + ((t / TILE_X) % TILE_Y),
             iz = tz * TILE_Z + t / (TILE_X * TILE_Y);
   CellState s;
+  CellTabs tb;
+  double zm = 0, zp = 0, delta = 0;
   bool active = ix < n && iy < n && iz < n;
   if (active) {
+    // every load of the cell (the grid's centre and stencil, the radial tables at r and r -+ h) depends only
+    // on the tile: issued together, before the workgroup's first barrier
+    const size_t cell = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
+    delta = (double)F[cell].x;
     cell_velocity<T>(m, F, ix, iy, iz, s);
-    s.dz = m.rsd ? s.vr * vgrowth_of_r(m.tab, s.r) : 0.0;
-    active = cell_may_bin(m, s);
+    cell_tabs(m, s, tb);
+    cell_z_range(m, s, zm, zp);
+    s.dz = m.rsd ? s.vr * tb.v : 0.0;
+    active = cell_may_bin(m, s, zm, zp);
   }
   if (!__syncthreads_or(active)) return;  // no cell of the tile reaches a channel
   for (int i = t; i < SLOTS; i += 256) {
-    keys[i] = ~0ull;
+    keys[i] = (K)~(K)0;
     vals[i] = (V)0;
   }
   __syncthreads();
   if (active) {
-    size_t cell = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
-    cell_hi(m, (double)F[cell].x, m.sig[0], s);
+    cell_hi_tabs(m, delta, m.sig[0], tb, s);
     double mass_sub = s.mass / m.n_sub;
-    V qsub = (V)0;
-    if constexpr (FX) qsub = __double2ull_rn(mass_sub * m.fxs[0]);
+    // the value a sample adds: its mK (FP) or its fixed-point mass (FX), from the cell's mass per sample
+    auto value = [&](double w, double pf) -> V {
+      if constexpr (FX) return (V)__double2ull_rn(w * m.fxs[0]);
+      else return (V)(w * pf);
+    };
     // npix = 12 Nside^2 < 2^32: the key is one 32 x 32 -> 64-bit multiply-add
-    auto add = [&](int inu, int64_t ipix, double pf) {
+    auto add_v = [&](int inu, int64_t ipix, V v) {
       if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix) {
-        V v;
-        if constexpr (FX) v = qsub;
-        else v = (T)(mass_sub * pf);
-        agg_add<V, SLOTS>((unsigned long long)(unsigned)inu * (unsigned)m.npix + (unsigned)ipix, v, keys, vals,
-                          vmaps);
+        const K key = (K)((unsigned long long)(unsigned)inu * (unsigned)m.npix + (unsigned)ipix);
+        agg_add<V, SLOTS, K>(key, home_slot<SLOTS, SEG, K>(key), v, keys, vals, vmaps);
       }
     };
+    auto add = [&](int inu, int64_t ipix, double pf) { add_v(inu, ipix, value(mass_sub, pf)); };
     if (m.nu_mode == 0 && m.scr.on) {
-      // two samples per iteration, screened: both channel loads in flight before either is resolved
-      const float dzf = (float)s.dz;
-      int isub = 0;
-      for (; isub + 1 < m.n_sub; isub += 2) {
-        SampleS qa, qb;
-        screen_channel(m, s, dzf, isub, qa);
-        screen_channel(m, s, dzf, isub + 1, qb);
-        int64_t pa, pb;
-        const int ia = resolve_sample(m, s, isub, qa, pa);
-        const int ib = resolve_sample(m, s, isub + 1, qb, pb);
-        add(ia, pa, qa.pf);
-        add(ib, pb, qb.pf);
-      }
-      if (isub < m.n_sub) {
+      CellScreen cs;
+      screen_cell(m, s, zm, zp, cs);
+      // the value of a sample in each window channel
+      auto win_value = [&](int c) -> V { return value(mass_sub, (c >= 0 && c < m.n_nu) ? m.prefac[c] : 0.0); };
+      const V v0 = win_value(cs.c0), v1 = win_value(cs.c0 + 1), v2 = win_value(cs.c0 + 2);
+      const int wv = t >> 6;
+      int qn = 0;  // entries in this wave's queue (the same in every lane)
+      for (int isub = 0; isub < m.n_sub; isub++) {
         SampleS q;
-        screen_channel(m, s, dzf, isub, q);
-        int64_t ipix;
-        const int inu = resolve_sample(m, s, isub, q, ipix);
-        add(inu, ipix, q.pf);
+        screen_channel(m, s, cs, isub, q);
+        const int p = q.inu >= 0 ? screen_pixel(m, q) : -1;
+        const bool d = q.inu == -2 || p == -2;
+        if (!d) add_v(q.inu, p, q.k == 2 ? v2 : (q.k == 1 ? v1 : v0));
+        const unsigned long long md = __ballot(d);
+        if (md) {
+          if (d) {
+            const int e = qn + lane_rank(md);
+            double x, y, z;
+            sample_pos(m, s, isub, x, y, z);
+            if (e < DEFER_CAP) {
+              dq.x[wv][e] = x;
+              dq.y[wv][e] = y;
+              dq.z[wv][e] = z;
+              dq.dz[wv][e] = s.dz;
+              dq.w[wv][e] = mass_sub;
+              dq.inu[wv][e] = q.inu;
+            }
+            else {  // queue full: the FP64 decision in this step
+              int64_t ipix;
+              double pf;
+              const int c = resolve_deferred(m, x, y, z, s.dz, q.inu, ipix, pf);
+              add(c, ipix, pf);
+            }
+          }
+          qn = min(qn + __popcll(md), DEFER_CAP);
+        }
       }
-    }
-    else if (m.nu_mode == 0) {
-      // two samples per iteration: their loads and FP64 chains overlap (62.4 against 64.7 ms at 1024^3)
-      int isub = 0;
-      for (; isub + 1 < m.n_sub; isub += 2) {
-        SamplePre qa, qb;
-        sample_pre<true>(m, s, isub, qa);
-        sample_pre<true>(m, s, isub + 1, qb);
-        int64_t pa, pb;
-        double fa, fb;
-        const int ia = sample_post<true>(m, qa, pa, fa);
-        const int ib = sample_post<true>(m, qb, pb, fb);
-        add(ia, pa, fa);
-        add(ib, pb, fb);
-      }
-      if (isub < m.n_sub) {
+      // resolve the queue: entry e by the e-th active lane (mod the number of active lanes)
+      const unsigned long long ex = __ballot(1);
+      const int nact = __popcll(ex), rank = lane_rank(ex);
+      for (int e = rank; e < qn; e += nact) {
         int64_t ipix;
         double pf;
-        const int inu = sample_bin_t<true>(m, s, isub, ipix, pf);
-        add(inu, ipix, pf);
+        const int inu = resolve_deferred(m, dq.x[wv][e], dq.y[wv][e], dq.z[wv][e], dq.dz[wv][e], dq.inu[wv][e],
+                                         ipix, pf);
+        if (inu >= 0) add_v(inu, ipix, value(dq.w[wv][e], pf));
       }
     }
     else {
@@ -461,9 +618,15 @@ __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T
   }
   __syncthreads();
   for (int i = t; i < SLOTS; i += 256) {
-    unsigned long long k = keys[i];
-    if (k != ~0ull) global_add(vmaps + k, vals[i]);
+    const K k = keys[i];
+    if (k != (K)~(K)0) global_add(vmaps + k, vals[i]);
   }
+}
+
+template <typename T, int SLOTS, int SEG, bool FX, typename K>
+__global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T *__restrict__ maps, MapParams m)
+{
+  pixelize_body<T, SLOTS, SEG, FX, K>(F, maps, m);
 }
 
 // Deterministic scatter, step 1: the box's HI mass (the same cell_hi as the map stage) summed in a fixed
@@ -668,8 +831,8 @@ __global__ __launch_bounds__(256) void k_psources(const cx<T> *__restrict__ F, M
   const int t = threadIdx.x;
   const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
   if (bid >= (size_t)m.ntx * m.nty * m.ntz) return;   // whole workgroup
-  const size_t tb = m.tile_order[bid];
-  const int tx = (int)(tb % m.ntx), ty = (int)((tb / m.ntx) % m.nty), tz = (int)(tb / ((size_t)m.ntx * m.nty));
+  const uint32_t tb = m.tile_order[bid];
+  const int tx = (int)(tb & 1023u), ty = (int)((tb >> 10) & 1023u), tz = (int)(tb >> 20);
   const int ix = tx * TILE_X + (t % TILE_X), iy = ty * TILE_Y + ((t / TILE_X) % TILE_Y),
             iz = tz * TILE_Z + t / (TILE_X * TILE_Y);
   for (int i = t; i < SLOTS; i += 256) {
@@ -795,11 +958,12 @@ __global__ __launch_bounds__(256) void k_sample_indices(const cx<T> *__restrict_
     CellState s;
     cell_velocity<T>(m, F, ix, iy, iz, s);
     cell_hi(m, (double)F[cell].x, m.sig[0], s);
-    const float dzf = (float)s.dz;
+    CellScreen cs;
+    decide_cell(m, s, cs);
     for (int isub = 0; isub < m.n_sub; isub++) {
       int64_t ipix;
       double pf;
-      int inu = sample_decide(m, s, dzf, isub, ipix, pf);
+      int inu = sample_decide(m, s, cs, isub, ipix, pf);
       inu_out[c * m.n_sub + isub] = inu;
       ipix_out[c * m.n_sub + isub] = ipix;
     }
@@ -875,20 +1039,23 @@ __global__ __launch_bounds__(256) void k_screen_check(const cx<T> *__restrict__ 
     CellState s;
     cell_velocity<T>(m, F, ix, iy, iz, s);
     cell_hi(m, (double)F[cell].x, m.sig[0], s);
-    const float dzf = (float)s.dz;
+    CellScreen cs;
+    decide_cell(m, s, cs);
     for (int isub = 0; isub < m.n_sub; isub++) {
       int64_t p64;
       double pf64;
       const int i64 = sample_bin(m, s, isub, p64, pf64);
       SampleS q;
-      screen_channel(m, s, dzf, isub, q);
+      screen_channel(m, s, cs, isub, q);
       nch += q.inu == -2;
       if (q.inu >= 0 && screen_pixel(m, q) == -2) npix++;
+      // a decided channel must be the FP64 one with the same factor as its window value
+      const bool chan_bad = q.inu >= 0 && (q.inu != i64 || m.prefac[q.inu] != pf64);
       int64_t pq;
       const int iq = resolve_sample(m, s, isub, q, pq);
       nsmp++;
       nbin += i64 >= 0;
-      nbad += (iq != i64) || (i64 >= 0 && (pq != p64 || q.pf != pf64));
+      nbad += chan_bad || (iq != i64) || (i64 >= 0 && pq != p64);
     }
   }
   atomicAdd(&red[0], nsmp);
@@ -931,9 +1098,26 @@ static void launch_pixelize(const cx<T> *F, T *maps, const MapParams &m, hipStre
   size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
   unsigned gx = (unsigned)(ntiles < 65536 ? ntiles : 65536);
   unsigned gy = (unsigned)((ntiles + gx - 1) / gx);
-  if (sc.pix == 1) hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS / 2, FX>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
-  else if (sc.pix == 2) hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS * 2, FX>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
-  else hipLaunchKernelGGL((k_pixelize<T, AGG_SLOTS, FX>), dim3(gx, gy), dim3(256), 0, s, F, maps, m);
+  // 32-bit bin keys when every key and the empty marker fit
+  const bool k32 = (uint64_t)m.n_nu * (uint64_t)m.npix < 0xFFFFFFFFull;
+  using K64 = unsigned long long;
+  const dim3 g(gx, gy), blk(256);
+  if (sc.pix == 1) {
+    if (k32) hipLaunchKernelGGL((k_pixelize<T, 2048, 3, FX, unsigned>), g, blk, 0, s, F, maps, m);
+    else hipLaunchKernelGGL((k_pixelize<T, 2048, 3, FX, K64>), g, blk, 0, s, F, maps, m);
+  }
+  else if (sc.pix == 2) {
+    if (k32) hipLaunchKernelGGL((k_pixelize<T, 4096, 2, FX, unsigned>), g, blk, 0, s, F, maps, m);
+    else hipLaunchKernelGGL((k_pixelize<T, 4096, 2, FX, K64>), g, blk, 0, s, F, maps, m);
+  }
+  else if (sc.pix == 3) {
+    if (k32) hipLaunchKernelGGL((k_pixelize<T, 1024, 2, FX, unsigned>), g, blk, 0, s, F, maps, m);
+    else hipLaunchKernelGGL((k_pixelize<T, 1024, 2, FX, K64>), g, blk, 0, s, F, maps, m);
+  }
+  else {
+    if (k32) hipLaunchKernelGGL((k_pixelize<T, 2048, 2, FX, unsigned>), g, blk, 0, s, F, maps, m);
+    else hipLaunchKernelGGL((k_pixelize<T, 2048, 2, FX, K64>), g, blk, 0, s, F, maps, m);
+  }
 }
 
 // ev[0] before clearing, ev[1] before the scatter, ev[2] after the last map kernel
