@@ -38,6 +38,7 @@ struct crime_gh {
   void *d_amp = nullptr;       // amplitude per |k|^2
   uint32_t *d_order = nullptr; // map-stage tiles by distance from the observer
   double *d_sub = nullptr;     // [3*n_sub]
+  int *d_subperm = nullptr;    // [n_sub] the map stage's visiting order of the offsets
   double *d_modes = nullptr;   // injected unit modes
   void *d_tw = nullptr;        // twiddles exp(+2 pi i m/N)
   double *d_partials = nullptr;
@@ -309,7 +310,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_chwin, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub,
+  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_chwin, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub, h->d_subperm,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig, h->d_ps, h->d_ps_count, h->ps_maps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -401,6 +402,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_edges, (d->n_nu + 1) * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_prefac, d->n_nu * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_sub, 3 * (size_t)d->n_sub * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_subperm, (size_t)d->n_sub * sizeof(int)));
   HIP_OR_FAIL(h, hipMalloc(&h->d_tw, h->n * 2 * esz));
   h->nparts = fields_partials_count(h->n, h->prec);
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_partials, 2 * (size_t)h->nparts * sizeof(double)));
@@ -420,6 +422,32 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   HIP_OR_FAIL(h, hipMemcpy(h->d_pk, d->logk, d->numk * sizeof(double), hipMemcpyHostToDevice));
   HIP_OR_FAIL(h, hipMemcpy(h->d_pk + d->numk, d->pk, d->numk * sizeof(double), hipMemcpyHostToDevice));
   HIP_OR_FAIL(h, hipMemcpy(h->d_sub, d->sub_off, 3 * (size_t)d->n_sub * sizeof(double), hipMemcpyHostToDevice));
+  {
+    // the map stage visits the offsets along a nearest-neighbour chain from the first: consecutive samples of
+    // a cell then often share a (channel, pixel) bin and are summed in registers before the LDS table
+    const int ns = d->n_sub;
+    std::vector<int> perm(ns);
+    std::vector<char> used(ns, 0);
+    perm[0] = 0;
+    used[0] = 1;
+    for (int j = 1; j < ns; j++) {
+      const double *a = d->sub_off + 3 * perm[j - 1];
+      int best = -1;
+      double bd = INFINITY;
+      for (int i = 0; i < ns; i++) {
+        if (used[i]) continue;
+        const double *b = d->sub_off + 3 * i;
+        const double dd = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+        if (dd < bd) {
+          bd = dd;
+          best = i;
+        }
+      }
+      perm[j] = best;
+      used[best] = 1;
+    }
+    HIP_OR_FAIL(h, hipMemcpy(h->d_subperm, perm.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+  }
 
   // channel edges and mK factors (pixelize.c:155, 246-256)
   std::vector<double> edges(d->n_nu + 1), prefac(d->n_nu);
@@ -541,6 +569,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   m.rsd = d->rsd;
   m.n_sub = d->n_sub;
   m.sub_off = h->d_sub;
+  m.sub_perm = h->d_subperm;
   double hmax = 0;
   for (int i = 0; i < d->n_sub; i++) {
     const double *o = d->sub_off + 3 * i;

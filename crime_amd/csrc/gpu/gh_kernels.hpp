@@ -160,6 +160,7 @@ struct MapParams {
   int rsd;
   int n_sub;
   const double *sub_off;   // [3*n_sub]
+  const int *sub_perm;     // [n_sub] the map stage's visiting order of the offsets (a nearest-neighbour chain)
   double h_sub;            // bound on |offset| (with rounding margin)
   int n_nu, nu_mode;
   const double *edges;     // [n_nu+1]

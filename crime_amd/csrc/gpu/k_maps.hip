@@ -115,6 +115,7 @@ __device__ __forceinline__ void cell_hi(const MapParams &m, double delta, double
 // constant address space they are scalar loads (with a generic pointer the atomics in the loop keep the
 // compiler from proving that, and every sample issued two vector loads).
 typedef const __attribute__((address_space(4))) double const_double;
+typedef const __attribute__((address_space(4))) int const_int;
 
 // Channel and pixel of one sample (pixelize.c:207-222), in two halves so that the per-sample loop can
 // interleave two samples: sample_pre computes the position, r, the frequency and loads the channel
@@ -443,12 +444,13 @@ __device__ __forceinline__ unsigned home_slot(K key)
 }
 
 template <typename T, int SLOTS, typename K>
-__device__ __forceinline__ void agg_add(K key, unsigned h, T val, K *keys, T *vals, T *__restrict__ maps)
+__device__ __forceinline__ void agg_add(K key, unsigned h, T val, K *keys, T *vals, T *__restrict__ maps,
+                                        int probes = 64)
 {
   const K EMPTY = (K)~(K)0;
   // the compare-and-swap claims an empty slot and returns the key of an occupied one: one LDS round trip
   // per probe
-  for (int p = 0; p < 64; p++) {
+  for (int p = 0; p < probes; p++) {
     const K old = atomicCAS(&keys[h], EMPTY, key);
     if (old == EMPTY || old == key) {
       atomicAdd(&vals[h], val);
@@ -457,6 +459,36 @@ __device__ __forceinline__ void agg_add(K key, unsigned h, T val, K *keys, T *va
     h = (h + 1) & (SLOTS - 1);
   }
   global_add(maps + key, val);
+}
+
+// agg_add in two halves, so that the first probe's round trip overlaps the next sample's arithmetic: start
+// issues the compare-and-swap at the home slot, finish (called later, before any other LDS access of this
+// thread to the table) adds the value there or continues probing from the next slot
+template <typename K, typename T>
+struct PendingAdd {
+  K key, old;
+  unsigned h;
+  T val;
+  bool on;
+};
+
+template <typename T, int SLOTS, typename K>
+__device__ __forceinline__ void agg_start(PendingAdd<K, T> &pa, K key, unsigned h, T val, K *keys)
+{
+  pa.old = atomicCAS(&keys[h], (K)~(K)0, key);
+  pa.key = key;
+  pa.h = h;
+  pa.val = val;
+  pa.on = true;
+}
+
+template <typename T, int SLOTS, typename K>
+__device__ __forceinline__ void agg_finish(PendingAdd<K, T> &pa, K *keys, T *vals, T *__restrict__ maps)
+{
+  if (!pa.on) return;
+  pa.on = false;
+  if (pa.old == (K)~(K)0 || pa.old == pa.key) atomicAdd(&vals[pa.h], pa.val);
+  else agg_add<T, SLOTS, K>(pa.key, (pa.h + 1) & (SLOTS - 1), pa.val, keys, vals, maps, 63);
 }
 
 // Samples whose FP32 decision missed a margin, queued per wave in LDS and resolved by the FP64 path
@@ -566,12 +598,27 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
       const V v0 = win_value(cs.c0), v1 = win_value(cs.c0 + 1), v2 = win_value(cs.c0 + 2);
       const int wv = t >> 6;
       int qn = 0;  // entries in this wave's queue (the same in every lane)
-      for (int isub = 0; isub < m.n_sub; isub++) {
+      // the current run of samples with one key: its value is summed in registers, its slot claimed when it
+      // starts and its sum added when it ends
+      PendingAdd<K, V> run;
+      run.on = false;
+      for (int j = 0; j < m.n_sub; j++) {
+        const int isub = ((const const_int *)m.sub_perm)[j];
         SampleS q;
         screen_channel(m, s, cs, isub, q);
         const int p = q.inu >= 0 ? screen_pixel(m, q) : -1;
         const bool d = q.inu == -2 || p == -2;
-        if (!d) add_v(q.inu, p, q.k == 2 ? v2 : (q.k == 1 ? v1 : v0));
+        if (!d && q.inu >= 0 && (uint64_t)p < (uint64_t)m.npix) {
+          const K key = (K)((unsigned long long)(unsigned)q.inu * (unsigned)m.npix + (unsigned)p);
+          const V v = q.k == 2 ? v2 : (q.k == 1 ? v1 : v0);
+          if (run.on && key == run.key) {
+            run.val += v;
+          }
+          else {
+            agg_finish<V, SLOTS, K>(run, keys, vals, vmaps);
+            agg_start<V, SLOTS, K>(run, key, home_slot<SLOTS, SEG, K>(key), v, keys);
+          }
+        }
         const unsigned long long md = __ballot(d);
         if (md) {
           if (d) {
@@ -596,6 +643,7 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
           qn = min(qn + __popcll(md), DEFER_CAP);
         }
       }
+      agg_finish<V, SLOTS, K>(run, keys, vals, vmaps);
       // resolve the queue: entry e by the e-th active lane (mod the number of active lanes)
       const unsigned long long ex = __ballot(1);
       const int nact = __popcll(ex), rank = lane_rank(ex);
