@@ -449,7 +449,8 @@ __device__ __forceinline__ void agg_add(K key, unsigned h, T val, K *keys, T *va
 {
   const K EMPTY = (K)~(K)0;
   // the compare-and-swap claims an empty slot and returns the key of an occupied one: one LDS round trip
-  // per probe
+  // per probe (unrolled by 8 only: a collision is rare, and a longer chain costs scalar registers)
+#pragma unroll 8
   for (int p = 0; p < probes; p++) {
     const K old = atomicCAS(&keys[h], EMPTY, key);
     if (old == EMPTY || old == key) {
@@ -673,6 +674,15 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
 
 template <typename T, int SLOTS, int SEG, bool FX, typename K>
 __global__ __launch_bounds__(256) void k_pixelize(const cx<T> *__restrict__ F, T *__restrict__ maps, MapParams m)
+{
+  pixelize_body<T, SLOTS, SEG, FX, K>(F, maps, m);
+}
+
+// the same with the register budget of 7 waves per SIMD (72 VGPRs), for the tables whose LDS footprint
+// lets 7 workgroups share a CU (32-bit keys, at most 2048 bins)
+template <typename T, int SLOTS, int SEG, bool FX, typename K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_pixelize_w7(
+    const cx<T> *__restrict__ F, T *__restrict__ maps, MapParams m)
 {
   pixelize_body<T, SLOTS, SEG, FX, K>(F, maps, m);
 }
@@ -1150,22 +1160,31 @@ static void launch_pixelize(const cx<T> *F, T *maps, const MapParams &m, hipStre
   const bool k32 = (uint64_t)m.n_nu * (uint64_t)m.npix < 0xFFFFFFFFull;
   using K64 = unsigned long long;
   const dim3 g(gx, gy), blk(256);
-  if (sc.pix == 1) {
-    if (k32) hipLaunchKernelGGL((k_pixelize<T, 2048, 3, FX, unsigned>), g, blk, 0, s, F, maps, m);
-    else hipLaunchKernelGGL((k_pixelize<T, 2048, 3, FX, K64>), g, blk, 0, s, F, maps, m);
-  }
-  else if (sc.pix == 2) {
-    if (k32) hipLaunchKernelGGL((k_pixelize<T, 4096, 2, FX, unsigned>), g, blk, 0, s, F, maps, m);
-    else hipLaunchKernelGGL((k_pixelize<T, 4096, 2, FX, K64>), g, blk, 0, s, F, maps, m);
-  }
-  else if (sc.pix == 3) {
-    if (k32) hipLaunchKernelGGL((k_pixelize<T, 1024, 2, FX, unsigned>), g, blk, 0, s, F, maps, m);
-    else hipLaunchKernelGGL((k_pixelize<T, 1024, 2, FX, K64>), g, blk, 0, s, F, maps, m);
-  }
-  else {
-    if (k32) hipLaunchKernelGGL((k_pixelize<T, 2048, 2, FX, unsigned>), g, blk, 0, s, F, maps, m);
-    else hipLaunchKernelGGL((k_pixelize<T, 2048, 2, FX, K64>), g, blk, 0, s, F, maps, m);
-  }
+  // FP32 maps with 32-bit keys and at most 2048 bins fit 7 workgroups per CU in LDS: the 7-wave kernel
+  constexpr bool W7 = sizeof(T) == 4 && !FX;
+  auto run = [&](auto slots, auto seg, bool w7) {
+    constexpr int S = decltype(slots)::value, G = decltype(seg)::value;
+    if (k32) {
+      if constexpr (W7 && S <= 2048) {
+        if (w7) {
+          hipLaunchKernelGGL((k_pixelize_w7<T, S, G, FX, unsigned>), g, blk, 0, s, F, maps, m);
+          return;
+        }
+      }
+      hipLaunchKernelGGL((k_pixelize<T, S, G, FX, unsigned>), g, blk, 0, s, F, maps, m);
+    }
+    else {
+      hipLaunchKernelGGL((k_pixelize<T, S, G, FX, K64>), g, blk, 0, s, F, maps, m);
+    }
+  };
+  using I2048 = std::integral_constant<int, 2048>;
+  using I1024 = std::integral_constant<int, 1024>;
+  using S2 = std::integral_constant<int, 2>;
+  using S3 = std::integral_constant<int, 3>;
+  if (sc.pix == 1) run(I2048{}, S3{}, true);
+  else if (sc.pix == 2) run(I2048{}, S2{}, false);
+  else if (sc.pix == 3) run(I1024{}, S2{}, true);
+  else run(I2048{}, S2{}, true);
 }
 
 // ev[0] before clearing, ev[1] before the scatter, ev[2] after the last map kernel
