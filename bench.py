@@ -24,9 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# VALU issue peak of the MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions per second
-# (a wave64 instruction occupies a 16-lane SIMD for 4 cycles; packed FP32 is not used)
-VALU_PEAK_LANE_INSTR = 256 * 4 * 16 * 2.4e9
+# VALU issue peak of the MI355X: its published FP32 vector peak, 157.3 TFLOP/s, is 7.86e13 lane-FMAs per
+# second = 256 CUs x 4 SIMDs x 32 lanes x 2.4 GHz (a wave64 FP32 instruction every 2 cycles per SIMD; FP64
+# and transcendental instructions take longer, so a mixed kernel stays below it)
+VALU_PEAK_LANE_INSTR = 256 * 4 * 32 * 2.4e9
 WORKLOADS = ("production_1024", "small_fp64_256", "largest_2048")
 METRIC = {"production_1024": "cells/s end-to-end at 1024^3",
           "small_fp64_256": "cells/s end-to-end at 256^3 (FP64)",
