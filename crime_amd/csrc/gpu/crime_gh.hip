@@ -42,7 +42,7 @@ struct crime_gh {
   double *d_modes = nullptr;   // injected unit modes
   void *d_tw = nullptr;        // twiddles exp(+2 pi i m/N)
   double *d_partials = nullptr;
-  double *d_sig = nullptr;     // [sigma2, mean]
+  double *d_sig = nullptr;     // [sigma2, mean, max |phi| of the grid (infinite when unknown)]
   double *h_sig = nullptr;     // pinned mirror
   int nparts = 0;
   FillParams fill{};
@@ -405,8 +405,8 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_subperm, (size_t)d->n_sub * sizeof(int)));
   HIP_OR_FAIL(h, hipMalloc(&h->d_tw, h->n * 2 * esz));
   h->nparts = fields_partials_count(h->n, h->prec);
-  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_partials, 2 * (size_t)h->nparts * sizeof(double)));
-  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_sig, 2 * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_partials, 3 * (size_t)h->nparts * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_sig, 3 * sizeof(double)));
   HIP_OR_FAIL(h, hipHostMalloc((void **)&h->h_sig, 2 * sizeof(double), hipHostMallocDefault));
   size_t nhalf = (size_t)h->n * h->n * (h->n / 2 + 1);
   if (d->mode_source == CRIME_GH_MODES_INJECTED) {
@@ -635,8 +635,14 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   m.fxmaps = h->d_fx;
   m.fxs = h->d_fxs;
   m.mass_part = h->d_fxs ? h->d_fxs + 2 : nullptr;
-  double zero2[2] = {0, 0};
-  HIP_OR_FAIL(h, hipMemcpy(h->d_sig, zero2, sizeof(zero2), hipMemcpyHostToDevice));
+  // tile culling before any grid load: |v_r| <= sqrt(3) max|phi| / dx for the central-difference
+  // velocities, so |dz| <= that times the largest |V(r)| of the table (with a rounding slack)
+  m.precull = d->velocity_mode == CRIME_GH_VEL_STENCIL;
+  double vmax = 0;
+  for (int i = 0; i < CRIME_GH_NZ; i++) vmax = fmax(vmax, fabs(d->v_r2z[i]));
+  m.dz_per_phi = d->rsd ? sqrt(3.0) * m.inv_dx * vmax * (1 + 1e-6) : 0.0;
+  double sig0[3] = {0, 0, INFINITY};
+  HIP_OR_FAIL(h, hipMemcpy(h->d_sig, sig0, sizeof(sig0), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -1152,6 +1158,9 @@ extern "C" int crime_gh_set_grid(crime_gh_t *h, int which, const double *src)
                 : launch_grid_set<float>((cx<float> *)grid, h->ncell, half, tmp, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(tmp);
+  // the grid's bound on |phi| no longer holds: the map stage culls by each cell's own shift only
+  const double inf = INFINITY;
+  if (e == hipSuccess) e = hipMemcpy(h->d_sig + 2, &inf, sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "set_grid: %s", hipGetErrorString(e));
   h->fields_done = true;
   return 0;

@@ -168,10 +168,12 @@ struct MapParams {
   double nu_lo, nu_hi;     // a sample can be binned only if nu_lo <= nu < nu_hi
   const double *prefac;    // [n_nu] mK factor per channel (pixelize.c:246-256)
   int64_t nside, npix;
-  const double *sig;       // device [sigma2, mean]
+  const double *sig;       // device [sigma2, mean, max |phi| of the grid or infinity]
   const ChanRec *crec;     // [n_guess] channel records of the buckets of [g_lo, g_lo + n_guess/g_inv) (table mode)
   int n_guess;
   double g_lo, g_inv;
+  int precull;                 // cull whole tiles before any grid load (central-difference velocities)
+  double dz_per_phi;           // |dz| <= dz_per_phi * sig[2] for every cell
   const uint32_t *tile_order;  // tiles by increasing distance of their centre from the observer, as
                                // tx | ty << 10 | tz << 20
   int ntx, nty, ntz;

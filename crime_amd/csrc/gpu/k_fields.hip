@@ -14,7 +14,8 @@
 //   pass 2  k_fft_y  : transform along ky
 //   pass 3  k_fft_x  : transform along kx (contiguous), scale by (sqrt(2 pi)/L)^3 (fourier.c:403),
 //                      FP64 partial sums of delta and delta^2 (compute_sigma_dens, fourier.c:24-76)
-//   k_sigma_finalize : fixed-order FP64 reduction of the partials -> sigma^2, <delta>
+//   k_sigma_finalize : fixed-order FP64 reduction of the partials -> sigma^2, <delta>, and the largest
+//                      |phi| of the grid (bounds the map stage's velocities)
 #include "gh_kernels.hpp"
 
 namespace crime {
@@ -196,22 +197,28 @@ __global__ __launch_bounds__((RowCfg<T, N, V>::NT)) void k_fft_x(cx<T> *__restri
   using C = RowCfg<T, N, V>;
   __shared__ cx<T> lds[N * C::TY];
   __shared__ double red[2 * C::NT];
+  __shared__ unsigned long long imax;
   const int t = threadIdx.x % C::TC, b = threadIdx.x / C::TC;
   const size_t row = (size_t)blockIdx.x * C::TY + b;
   cx<T> *line = F + row * N;
-  double s1 = 0, s2 = 0;
+  double s1 = 0, s2 = 0, am = 0;
+  if (threadIdx.x == 0) imax = 0;
+  __syncthreads();
   auto load = [&](int e) { return line[e]; };
   auto store = [&](int e, cx<T> v) {
     cx<T> o = {(T)((double)v.x * norm), (T)((double)v.y * norm_phi)};
     double d = (double)o.x;
     s1 += d;
     s2 += d * d;
+    am = fmax(am, fabs((double)o.y));
     line[e] = o;
   };
   auto slot = [&](int e) -> cx<T> & { return lds[b * N + e]; };
   Stockham<T, N, C::TC, C::MAXR>::run(t, load, store, slot, tw);
   red[threadIdx.x] = s1;
   red[C::NT + threadIdx.x] = s2;
+  // largest |imaginary part| of the block: non-negative doubles order as their bit patterns
+  atomicMax(&imax, (unsigned long long)__double_as_longlong(am));
   __syncthreads();
   for (int w = C::NT / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
@@ -223,6 +230,7 @@ __global__ __launch_bounds__((RowCfg<T, N, V>::NT)) void k_fft_x(cx<T> *__restri
   if (threadIdx.x == 0) {
     partials[2 * blockIdx.x] = red[0];
     partials[2 * blockIdx.x + 1] = red[C::NT];
+    partials[2 * gridDim.x + blockIdx.x] = __longlong_as_double((long long)imax);
   }
 }
 
@@ -324,19 +332,22 @@ template hipError_t launch_power_spectrum<double>(int, cx<double> *, const cx<do
 __global__ __launch_bounds__(1024) void k_sigma_finalize(const double *__restrict__ partials, int nparts,
                                                          double ncells, double *__restrict__ out)
 {
-  __shared__ double r1[1024], r2[1024];
-  double a = 0, b = 0;
+  __shared__ double r1[1024], r2[1024], r3[1024];
+  double a = 0, b = 0, c = 0;
   for (int i = threadIdx.x; i < nparts; i += 1024) {
     a += partials[2 * i];
     b += partials[2 * i + 1];
+    c = fmax(c, partials[2 * nparts + i]);
   }
   r1[threadIdx.x] = a;
   r2[threadIdx.x] = b;
+  r3[threadIdx.x] = c;
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       r1[threadIdx.x] += r1[threadIdx.x + w];
       r2[threadIdx.x] += r2[threadIdx.x + w];
+      r3[threadIdx.x] = fmax(r3[threadIdx.x], r3[threadIdx.x + w]);
     }
     __syncthreads();
   }
@@ -344,6 +355,7 @@ __global__ __launch_bounds__(1024) void k_sigma_finalize(const double *__restric
     double mean = r1[0] / ncells;
     out[0] = r2[0] / ncells - mean * mean;  // sigma2_gauss (fourier.c:74)
     out[1] = mean;
+    out[2] = r3[0];  // max |imaginary part| of the grid (the map stage's bound on the velocity)
   }
 }
 

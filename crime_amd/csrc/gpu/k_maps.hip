@@ -225,6 +225,28 @@ __device__ __forceinline__ float2 chan_window(const MapParams &m, int c)
   return m.scr.chwin[c];
 }
 
+// true when no sample of any cell of tile (tx, ty, tz) can land in a channel, with every cell's RSD shift
+// within dzb: the tile's box of cell centres, widened by h_sub, spans the radii [rlo, rhi], so every
+// sample's redshift lies in [z(rlo) - dzb, z(rhi) + dzb] (z(r) is non-decreasing); the same test as
+// cell_may_bin on that range.  dzb infinite (or NaN): false.
+__device__ __forceinline__ bool tile_out_of_band(const MapParams &m, int tx, int ty, int tz, double dzb)
+{
+  if (!(dzb < INFINITY)) return false;
+  const int t0[3] = {tx * TILE_X, ty * TILE_Y, tz * TILE_Z}, tn[3] = {TILE_X, TILE_Y, TILE_Z};
+  double lo2 = 0, hi2 = 0;
+  for (int d = 0; d < 3; d++) {
+    const double a = m.dx * (t0[d] + 0.5) - m.obs[d], b = m.dx * (t0[d] + tn[d] - 0.5) - m.obs[d];
+    const double nearest = (a <= 0 && b >= 0) ? 0.0 : fmin(fabs(a), fabs(b));
+    const double farthest = fmax(fabs(a), fabs(b));
+    lo2 += nearest * nearest;
+    hi2 += farthest * farthest;
+  }
+  const double zl = z_of_r(m.tab, sqrt(lo2) - m.h_sub) - dzb, zh = z_of_r(m.tab, sqrt(hi2) + m.h_sub) + dzb;
+  if (1 + zl > 0 && 1420.40575177 * (1 + 1e-10) < m.nu_lo * (1 + zl)) return true;
+  if (1 + zh > 0 && 1420.40575177 * (1 - 1e-10) >= m.nu_hi * (1 + zh)) return true;
+  return false;
+}
+
 // false when no sample of the cell can land in a channel (conservative); zm, zp as cell_z_range
 __device__ __forceinline__ bool cell_may_bin(const MapParams &m, const CellState &s, double zm, double zp)
 {
@@ -552,6 +574,8 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
   if (bid >= (size_t)m.ntx * m.nty * m.ntz) return;
   const uint32_t b = m.tile_order[bid];
   const int tx = (int)(b & 1023u), ty = (int)((b >> 10) & 1023u), tz = (int)(b >> 20);
+  // whole tiles out of the band, by the grid's bound on the RSD shift: before any grid load
+  if (m.precull && tile_out_of_band(m, tx, ty, tz, m.dz_per_phi * m.sig[2])) return;
   const int ix = tx * TILE_X + (t % TILE_X), iy = ty * TILE_Y + ((t / TILE_X) % TILE_Y),
             iz = tz * TILE_Z + t / (TILE_X * TILE_Y);
   CellState s;
