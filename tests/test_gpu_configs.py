@@ -78,5 +78,5 @@ def test_config5_emulator():
           f"{t_mock / t_eval:.0f}x the mock rate", flush=True)
     assert losses[-200:].mean() < 0.05 * losses[:20].mean()
     assert e["rms_debiased"] < 0.5 * n["rms_debiased"]
-    assert e["rms_debiased"] < 0.3
+    assert e["rms_debiased"] < 0.2  # the target (measured 0.191; the same on every run)
     assert t_mock / t_eval > 1000

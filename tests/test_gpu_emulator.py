@@ -2,15 +2,14 @@
 on Latin-hypercube cosmologies -> binned C_l of 8 channel pairs -> the network trained on the GPU ->
 held-out cosmologies judged against fresh mock ensembles.
 
-What the emulator must show: a smaller error (in units of one realisation's cosmic variance, the noise
-of the held-out truth removed in quadrature) than the method without it, the C_l of the nearest training
-cosmology, and a median error under 0.4 sigma.  512 cosmologies x 8 realisations train it, as at full
+What the emulator must show: an error (in units of one realisation's cosmic variance, the noise of the
+held-out truth removed in quadrature) under 0.3 sigma RMS and 0.3 sigma median, and under half that of
+the method without it, the C_l of the nearest training cosmology.  512 cosmologies x 8 realisations train it, as at full
 size; 16 held-out cosmologies x 16 realisations judge it.
 
-The RMS over the 2048 held-out values is set by a few outliers and moves from run to run: the map
-scatter's FP32 atomics sum in a different order each run, so the training targets differ in their last
-bits, and 20 000 Adam steps on minibatches carry that into different weights.  The bound on the RMS is
-therefore the comparison itself (below the nearest-cosmology method), not a fixed fraction of it.
+The mocks use the fixed-point (deterministic) scatter and the training is deterministic, so the errors
+are the same numbers on every run (measured: RMS 0.249 sigma after removing the truth's noise, median
+0.217, against 0.80 for the nearest cosmology): the bounds below are fixed.
 """
 import time
 
@@ -45,7 +44,8 @@ def test_emulator_small_end_to_end():
     print(f"simulation {t_sim:.1f} s ({t_mock * 1e3:.2f} ms/mock); {res}; "
           f"emulator {t_eval * 1e6:.2f} us/cosmology = {t_mock / t_eval:.0f}x the mock rate")
     e, n = res["emulator"], res["nearest_cosmology"]
-    assert e["rms_debiased"] < n["rms_debiased"]
-    assert e["median_abs"] < 0.4
+    assert e["rms_debiased"] < 0.3
+    assert e["rms_debiased"] < 0.5 * n["rms_debiased"]
+    assert e["median_abs"] < 0.3
     # a 64^3 mock takes under a millisecond; the full-size rate (256^3 mocks) is measured by the bench
     assert t_mock / t_eval > 100
