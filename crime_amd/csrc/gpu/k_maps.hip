@@ -332,6 +332,8 @@ __device__ __forceinline__ void screen_channel(const MapParams &m, const CellSta
 // RING pixel from FP32, or -2 when a margin is not met (see vec2pix_ring for the FP64 form); npix < 2^31
 __device__ __forceinline__ int screen_pixel(const MapParams &m, const SampleS &q)
 {
+  // every margin test feeds one flag (no early exits: a wave stays on one path), and the folds of ip into
+  // [0, 4 ir) are one conditional add or subtract (|ip| stays within one period either side)
   const ScreenParams &c = m.scr;
   const int nside = (int)m.nside;
   const float N = (float)nside;
@@ -340,15 +342,16 @@ __device__ __forceinline__ int screen_pixel(const MapParams &m, const SampleS &q
   const float ax = fabsf(q.xf), ay = fabsf(q.yf);
   const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
   // on a coordinate plane atan2's signed-zero conventions decide: FP64
-  if (!(mn > 0.0f) || fabsf(za - (2.0f / 3.0f)) < c.eps_z) return -2;
+  bool ok = mn > 0.0f && fabsf(za - (2.0f / 3.0f)) >= c.eps_z;
   const float u = atan01(mn * __builtin_amdgcn_rcpf(mx)) * 0.63661977236758134f;  // quarter turns, [0, 0.5]
   const float w = ay > ax ? 1.0f - u : u;  // angle of (|x|, |y|) in quarter turns, [0, 1]
-  if (w < c.eps_ax || w > 1.0f - c.eps_ax) return -2;
+  ok = ok && w >= c.eps_ax && w <= 1.0f - c.eps_ax;
   // tt = phi / (pi/2) in [0, 4) = B + sw
   const bool xneg = q.xf < 0.0f, yneg = q.yf < 0.0f;
   const int B = xneg ? 2 : (yneg ? 4 : 0);
   const float sw = xneg != yneg ? -w : w;
   const float eps = c.eps_pix;
+  int pix;
   if (za < 2.0f / 3.0f) {
     // jp, jm = floor(N (0.5 + tt) -+ 0.75 N cth) = K0 + floor(h + N sw -+ 0.75 N cth), N (0.5 + B) = K0 + h
     const int twoK = nside * (1 + 2 * B);
@@ -357,32 +360,33 @@ __device__ __forceinline__ int screen_pixel(const MapParams &m, const SampleS &q
     const float v1 = (nw - nc) + h, v2 = (nw + nc) + h;
     const float f1 = floorf(v1), f2 = floorf(v2);
     const float d1 = v1 - f1, d2 = v2 - f2;
-    if (d1 < eps || d1 > 1.0f - eps || d2 < eps || d2 > 1.0f - eps) return -2;
+    ok = ok && d1 >= eps && d1 <= 1.0f - eps && d2 >= eps && d2 <= 1.0f - eps;
     const int jp = (twoK >> 1) + (int)f1, jm = (twoK >> 1) + (int)f2;
     const int ir = nside + 1 + jp - jm;
     const int kshift = 1 - (ir & 1);
-    int ip = (jp + jm - nside + kshift + 1) / 2;
-    ip = imod_small(ip, 4 * nside);
-    return nside * (nside - 1) * 2 + (ir - 1) * 4 * nside + ip;
+    int ip = (jp + jm - nside + kshift + 1) / 2;  // in [-Nside/2, 5 Nside + 1]
+    ip = ip < 0 ? ip + 4 * nside : (ip >= 4 * nside ? ip - 4 * nside : ip);
+    pix = nside * (nside - 1) * 2 + (ir - 1) * 4 * nside + ip;
   }
-  // polar caps: tp = tt - (int)tt, jp = floor(tp tmp), jm = floor((1 - tp) tmp), tmp = N sin(theta) sqrt(3/(1+za))
-  const float tp = sw > 0.0f ? w : 1.0f - w;
-  const float sth = __builtin_amdgcn_sqrtf(fmaf(q.xf, q.xf, q.yf * q.yf)) * __builtin_amdgcn_rcpf(q.rf);
-  const float tmp = N * sth * __builtin_amdgcn_rsqf((1.0f + za) * (1.0f / 3.0f));
-  const float p1 = tp * tmp, p2 = (1.0f - tp) * tmp;
-  const float g1 = floorf(p1), g2 = floorf(p2);
-  const float d1 = p1 - g1, d2 = p2 - g2;
-  if (d1 < eps || d1 > 1.0f - eps || d2 < eps || d2 > 1.0f - eps) return -2;
-  const int ir = (int)g1 + (int)g2 + 1;
-  // ip = floor(tt ir) = B ir + floor(sw ir)
-  const float p3 = sw * (float)ir;
-  const float g3 = floorf(p3);
-  const float d3 = p3 - g3;
-  if (d3 < eps || d3 > 1.0f - eps) return -2;
-  int ip = B * ir + (int)g3;
-  ip = imod_small(ip, 4 * ir);
-  if (cth > 0.0f) return 2 * ir * (ir - 1) + ip;
-  return 12 * nside * nside - 2 * ir * (ir + 1) + ip;
+  else {
+    // polar caps: tp = tt - (int)tt, jp = floor(tp tmp), jm = floor((1 - tp) tmp), tmp = N sin(theta) sqrt(3/(1+za))
+    const float tp = sw > 0.0f ? w : 1.0f - w;
+    const float sth = __builtin_amdgcn_sqrtf(fmaf(q.xf, q.xf, q.yf * q.yf)) * __builtin_amdgcn_rcpf(q.rf);
+    const float tmp = N * sth * __builtin_amdgcn_rsqf((1.0f + za) * (1.0f / 3.0f));
+    const float p1 = tp * tmp, p2 = (1.0f - tp) * tmp;
+    const float g1 = floorf(p1), g2 = floorf(p2);
+    const float d1 = p1 - g1, d2 = p2 - g2;
+    const int ir = (int)g1 + (int)g2 + 1;
+    // ip = floor(tt ir) = B ir + floor(sw ir)
+    const float p3 = sw * (float)ir;
+    const float g3 = floorf(p3);
+    const float d3 = p3 - g3;
+    ok = ok && d1 >= eps && d1 <= 1.0f - eps && d2 >= eps && d2 <= 1.0f - eps && d3 >= eps && d3 <= 1.0f - eps;
+    int ip = B * ir + (int)g3;  // in [-ir, 5 ir]
+    ip = ip < 0 ? ip + 4 * ir : (ip >= 4 * ir ? ip - 4 * ir : ip);
+    pix = cth > 0.0f ? 2 * ir * (ir - 1) + ip : 12 * nside * nside - 2 * ir * (ir + 1) + ip;
+  }
+  return ok ? pix : -2;
 }
 
 // finish a screened sample in its step: undecided channels take the whole FP64 path, undecided pixels
