@@ -34,11 +34,11 @@ struct crime_gh {
   double *d_edges = nullptr;   // [n_nu+1]
   double *d_prefac = nullptr;  // [n_nu]
   ChanRec *d_crec = nullptr;   // channel records of the frequency buckets
-  float2 *d_chwin = nullptr;   // [n_nu] decided frequency interval of each channel (FP32 screen)
+  float2 *d_chwin = nullptr;   // [n_nu] decided interval of 1 + z of each channel (FP32 screen)
   void *d_amp = nullptr;       // amplitude per |k|^2
   uint32_t *d_order = nullptr; // map-stage tiles by distance from the observer
   double *d_sub = nullptr;     // [3*n_sub]
-  int *d_subperm = nullptr;    // [n_sub] the map stage's visiting order of the offsets
+  double *d_subp = nullptr;    // [3*n_sub] the offsets in the map stage's visiting order
   double *d_modes = nullptr;   // injected unit modes
   void *d_tw = nullptr;        // twiddles exp(+2 pi i m/N)
   double *d_partials = nullptr;
@@ -173,9 +173,10 @@ static float f32_down(double x)
 //    floor(2h / dr) + 1 consecutive nodes of the channels' radii, with kb the run's mid-range.  The FP32
 //    evaluation: |p| = sqrt((float)|p|^2_FP64) within 1.25 u of |p| and the rounded r_c within 0.5 u,
 //    which moves z by at most 2 A u with A = r dz/dr; the roundings of z_c + dz, a, b, the two fmas and
-//    1 + z add at most 2 (1 + z) u; rcp, the product and the float NU_21 2.5 u relative.  So
-//    |dnu| <= nu (u (2 A / (1 + z) + 4.5) + E_q / (1 + z)), taken at the worst table node of the
-//    channels' redshift range, doubled;
+//    1 + z add at most 2 (1 + z) u (2.5 u more are allowed for: an earlier form took the frequency in
+//    FP32).  So NU_21 / (1 + z) in FP32 arithmetic is within |dnu| <= nu (u (2 A / (1 + z) + 4.5) +
+//    E_q / (1 + z)) of the exact frequency, taken at the worst table node of the channels' redshift range,
+//    doubled; the decided windows bound 1 + z so that NU_21 / (1 + z) keeps eps_nu inside the edges;
 //  - pixel: the angle (input rounding 0.32 u, t = min/max 0.48 u, polynomial and its evaluation 1.2 u,
 //    scalings 0.75 u) within 2.8 u quarter turns; cos(theta) within 3.3 u; sin(theta) and the cap's
 //    square root within 5 u relative; every floor argument of vec2pix is then within 8 N u of the exact
@@ -229,23 +230,32 @@ static int screen_setup(crime_gh *h, const crime_gh_desc *d, const std::vector<d
     worst = fmax(worst, nu * (u * (2 * A / (1 + zb) + 4.5) + e_q / (1 + za)));
   }
   const double eps_nu = 2 * fmax(worst, nu_hi * 4.5 * u);
-  // decided intervals [edge + eps, next edge - eps), rounded inwards
+  // decided intervals of 1 + z: NU_21 / (1 + z) in [edge + eps, next edge - eps], rounded inwards (the
+  // quotients widened by 1e-15 first, more than their FP64 rounding)
+  auto g_of = [](double nu) { return CRIME_GH_NU_21 / nu; };
   std::vector<float2> w(d->n_nu);
-  for (int k = 0; k < d->n_nu; k++) w[k] = make_float2(f32_up(edges[k] + eps_nu), f32_down(edges[k + 1] - eps_nu));
+  for (int k = 0; k < d->n_nu; k++)
+    w[k] = make_float2(f32_up(g_of(edges[k + 1] - eps_nu) * (1 + 1e-15)), f32_down(g_of(edges[k] + eps_nu) * (1 - 1e-15)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_chwin, w.size() * sizeof(float2)));
   HIP_OR_FAIL(h, hipMemcpy(h->d_chwin, w.data(), w.size() * sizeof(float2), hipMemcpyHostToDevice));
   c.chwin = h->d_chwin;
-  // the cloud [r_c - h, r_c + h] stays inside (0, r_last): no clamp of the table lerp inside it
-  c.rc_lo = hs * (1 + 1e-9) + 1e-6;
+  // the cloud [r_c - h, r_c + h] stays inside (r_min, r_last): no clamp of the table lerp inside it, and
+  // every sample's |p| is above r_min (radii below it are left to FP64)
+  const double r_min = 1e-3 * d->l_box / d->n_grid;
+  c.rc_lo = hs * (1 + 1e-9) + 1e-6 + 2 * r_min;
   c.rc_hi = d->r_r2z[CRIME_GH_NZ - 1] - hs * (1 + 1e-9) - 1e-6;
   c.inv_2h = 1 / (2 * hs);
   c.inv_2h2 = 1 / (2 * hs * hs);
-  c.r_min = (float)(1e-3 * d->l_box / d->n_grid);
-  c.out_lo = f32_down(nu_lo - eps_nu);
-  c.out_hi = f32_up(nu_hi + eps_nu);
+  // no channel: NU_21 / (1 + z) above nu_hi + eps or below nu_lo - eps
+  c.out_lo = f32_down(g_of(nu_hi + eps_nu) * (1 - 1e-15));
+  c.out_hi = nu_lo - eps_nu > 0 ? f32_up(g_of(nu_lo - eps_nu) * (1 + 1e-15)) : INFINITY;
   c.eps_pix = (float)(16 * (double)d->n_side * u);
   c.eps_ax = (float)(8 * u);
   c.eps_z = (float)(8 * u);
+  // |fl(d - 1/2)| <= half implies |d - 1/2| <= 1/2 - eps: d - 1/2 rounds by at most 2^-26 and d itself
+  // (a difference of floats in [0, 1]) by at most 2^-25
+  c.half_pix = f32_down(0.5 - c.eps_pix - 1.0 / 16777216.0);
+  c.half_ax = f32_down(0.5 - c.eps_ax - 1.0 / 16777216.0);
   h->eps_nu = eps_nu;
   return 0;
 }
@@ -310,7 +320,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_chwin, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub, h->d_subperm,
+  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_chwin, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub, h->d_subp,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig, h->d_ps, h->d_ps_count, h->ps_maps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -402,7 +412,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_edges, (d->n_nu + 1) * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_prefac, d->n_nu * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_sub, 3 * (size_t)d->n_sub * sizeof(double)));
-  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_subperm, (size_t)d->n_sub * sizeof(int)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_subp, 3 * (size_t)d->n_sub * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc(&h->d_tw, h->n * 2 * esz));
   h->nparts = fields_partials_count(h->n, h->prec);
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_partials, 3 * (size_t)h->nparts * sizeof(double)));
@@ -446,7 +456,10 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
       perm[j] = best;
       used[best] = 1;
     }
-    HIP_OR_FAIL(h, hipMemcpy(h->d_subperm, perm.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<double> offp(3 * (size_t)ns);
+    for (int j = 0; j < ns; j++)
+      for (int k = 0; k < 3; k++) offp[3 * (size_t)j + k] = d->sub_off[3 * (size_t)perm[j] + k];
+    HIP_OR_FAIL(h, hipMemcpy(h->d_subp, offp.data(), offp.size() * sizeof(double), hipMemcpyHostToDevice));
   }
 
   // channel edges and mK factors (pixelize.c:155, 246-256)
@@ -569,7 +582,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   m.rsd = d->rsd;
   m.n_sub = d->n_sub;
   m.sub_off = h->d_sub;
-  m.sub_perm = h->d_subperm;
+  m.sub_offp = h->d_subp;
   double hmax = 0;
   for (int i = 0; i < d->n_sub; i++) {
     const double *o = d->sub_off + 3 * i;

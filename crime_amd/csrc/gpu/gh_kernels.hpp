@@ -138,15 +138,15 @@ hipError_t launch_amp_table(const FillParams &f, void *tab, hipStream_t s);
 // from the FP64 path (the reference's arithmetic) otherwise; both give the same integers.
 struct ScreenParams {
   int on;                  // 0: every decision in FP64
-  const float2 *chwin;     // [n_nu] decided interval of each channel: its edges moved inwards by the
-                           // frequency margin, rounded inwards to float
-  double rc_lo, rc_hi;     // cells screened: rc_lo < r_c < rc_hi (the cloud keeps off r = 0 and the table end)
+  const float2 *chwin;     // [n_nu] decided interval [lo, hi] of 1 + z for each channel: NU_21 / (1 + z) at
+                           // least the frequency margin inside the channel's edges, rounded inwards to float
+  double rc_lo, rc_hi;     // cells screened: rc_lo < r_c < rc_hi (the cloud keeps off r = r_min and the table end)
   double inv_2h, inv_2h2;  // 1 / (2 h_sub), 1 / (2 h_sub^2): the per-cell parabola of z(r)
-  float r_min;             // radii below r_min are left to FP64
-  float out_lo, out_hi;    // nu < out_lo or nu >= out_hi: no channel (nu_lo - eps, nu_hi + eps)
+  float out_lo, out_hi;    // 1 + z < out_lo or > out_hi: no channel (nu > nu_hi + eps or nu < nu_lo - eps)
   float eps_pix;           // margin of the pixel floors, in the units of their arguments
   float eps_ax;            // margin of the quadrant / octant boundaries (w near 0 or 1), quarter turns
   float eps_z;             // margin of |cos theta| around 2/3
+  float half_pix, half_ax; // |d - 1/2| <= half (FP32) implies eps <= d <= 1 - eps, for the two margins above
 };
 
 struct MapParams {
@@ -160,7 +160,7 @@ struct MapParams {
   int rsd;
   int n_sub;
   const double *sub_off;   // [3*n_sub]
-  const int *sub_perm;     // [n_sub] the map stage's visiting order of the offsets (a nearest-neighbour chain)
+  const double *sub_offp;  // [3*n_sub] the offsets in the map stage's visiting order (a nearest-neighbour chain)
   double h_sub;            // bound on |offset| (with rounding margin)
   int n_nu, nu_mode;
   const double *edges;     // [n_nu+1]

@@ -189,9 +189,10 @@ __device__ __forceinline__ int sample_bin(const MapParams &m, const CellState &s
 //
 // The channel needs no memory access per sample.  Per cell: z(r) over the sample cloud (|r - r_c| <= h)
 // is the parabola through the FP64 table lerp at r_c - h, r_c and r_c + h, whose distance from the lerp
-// the host bounds and adds to the margin; and the decided frequency intervals [lo, hi) of the three
-// channels around the centre's channel (CellScreen).  Per sample: the parabola, 1 + z, a reciprocal
-// and two comparisons give the window slot k of the channel.
+// the host bounds and adds to the margin; and the decided intervals [lo, hi] of 1 + z of the three
+// channels around the centre's channel (CellScreen).  Per sample: the parabola, 1 + z and three interval
+// tests give the window slot k of the channel.  Margin tests |d - 1/2| <= half imply eps <= d <= 1 - eps
+// (the host takes half 2^-24 below 1/2 - eps, more than the rounding of d - 1/2).
 // ---------------------------------------------------------------------------------------------
 struct SampleS {
   float xf, yf, zf, rf;   // position relative to the observer (the reference's FP64 sums) in FP32, and |p|
@@ -201,8 +202,8 @@ struct SampleS {
 
 struct CellScreen {
   float rc, zcd, a, b;                 // z(r_c + t) + dz = zcd + t (a + b t); zcd NaN: no screen for the cell
-  float lo0, hi0, lo1, hi1, lo2, hi2;  // decided intervals of channels c0, c0 + 1, c0 + 2 (empty outside),
-                                       // disjoint and in increasing order
+  float lo0, hi0, lo1, hi1, lo2, hi2;  // decided intervals of 1 + z of channels c0, c0 + 1, c0 + 2 (empty
+                                       // outside the channels), disjoint
   int c0;
 };
 
@@ -216,12 +217,10 @@ __device__ __forceinline__ void sample_pos(const MapParams &m, const CellState &
   z = s.z0 + so[3 * isub + 2];
 }
 
-// decided interval of channel c: empty below channel 0 (-inf, -inf) and above the last (+inf, +inf), so
-// that the lower ends stay ordered across the window
+// decided interval of 1 + z for channel c; empty (+inf, -inf) below channel 0 and above the last
 __device__ __forceinline__ float2 chan_window(const MapParams &m, int c)
 {
-  if (c < 0) return make_float2(-INFINITY, -INFINITY);
-  if (c >= m.n_nu) return make_float2(INFINITY, INFINITY);
+  if (c < 0 || c >= m.n_nu) return make_float2(INFINITY, -INFINITY);
   return m.scr.chwin[c];
 }
 
@@ -265,7 +264,7 @@ __device__ __forceinline__ void screen_cell(const MapParams &m, const CellState 
                                             CellScreen &cs)
 {
   const ScreenParams &c = m.scr;
-  const bool ok = s.r > c.rc_lo && s.r < c.rc_hi;  // the cloud keeps off r = 0 and the table's end
+  const bool ok = s.r > c.rc_lo && s.r < c.rc_hi;  // the cloud keeps off r <= r_min and the table's end
   cs.rc = (float)s.r;
   // a cell none of whose samples can reach a channel: z = inf gives nu = 0, no channel for every sample
   // (the FP64 path's decision); otherwise NaN leaves every sample of an unscreened cell to FP64
@@ -304,29 +303,34 @@ __device__ __forceinline__ float atan01(float t)
 
 // position, |p| and the channel from FP32 (TABLE mode): q.inu = channel, -1 (no channel) or -2 (left to
 // FP64), q.k its window slot
-__device__ __forceinline__ void screen_channel(const MapParams &m, const CellState &s, const CellScreen &cs, int isub,
-                                               SampleS &q)
+__device__ __forceinline__ void screen_channel_at(const MapParams &m, double x, double y, double z, const CellScreen &cs,
+                                                  SampleS &q)
 {
   const ScreenParams &c = m.scr;
-  double x, y, z;
-  sample_pos(m, s, isub, x, y, z);
   q.xf = (float)x;
   q.yf = (float)y;
   q.zf = (float)z;
-  // |p|^2 in FP64 (exact to 1e-16), its root in FP32: |p| within 1.25 ulp
-  q.rf = __builtin_amdgcn_sqrtf((float)(x * x + y * y + z * z));
+  // |p|^2 in FP64 (within 4e-16), its root in FP32: |p| within 1.25 ulp
+  q.rf = __builtin_amdgcn_sqrtf((float)fma(x, x, fma(y, y, z * z)));
   const float t = q.rf - cs.rc;
   const float zs = fmaf(t, fmaf(t, cs.b, cs.a), cs.zcd);
-  const float nu = 1420.40575177f * __builtin_amdgcn_rcpf(1.0f + zs);
-  // three interval tests (a select among the struct's fields by k would leave the struct in memory)
-  const bool d0 = nu >= cs.lo0 && nu < cs.hi0;
-  const bool d1 = nu >= cs.lo1 && nu < cs.hi1;
-  const bool d2 = nu >= cs.lo2 && nu < cs.hi2;
+  // the channel from 1 + z against the windows' bounds on 1 + z (the frequency needs no reciprocal); three
+  // interval tests (a select among the struct's fields by k would leave the struct in memory)
+  const float g = 1.0f + zs;
+  const bool d0 = g >= cs.lo0 && g <= cs.hi0;
+  const bool d1 = g >= cs.lo1 && g <= cs.hi1;
+  const bool d2 = g >= cs.lo2 && g <= cs.hi2;
   const int k = d1 ? 1 : (d2 ? 2 : 0);
-  int inu = (d0 || d1 || d2) ? cs.c0 + k : ((nu < c.out_lo || nu >= c.out_hi) ? -1 : -2);
-  if (!(q.rf > c.r_min)) inu = -2;  // near the observer
-  q.inu = inu;
+  q.inu = (d0 || d1 || d2) ? cs.c0 + k : ((g < c.out_lo || g > c.out_hi) ? -1 : -2);
   q.k = k;
+}
+
+__device__ __forceinline__ void screen_channel(const MapParams &m, const CellState &s, const CellScreen &cs, int isub,
+                                               SampleS &q)
+{
+  double x, y, z;
+  sample_pos(m, s, isub, x, y, z);
+  screen_channel_at(m, x, y, z, cs, q);
 }
 
 // RING pixel from FP32, or -2 when a margin is not met (see vec2pix_ring for the FP64 form); npix < 2^31
@@ -345,28 +349,31 @@ __device__ __forceinline__ int screen_pixel(const MapParams &m, const SampleS &q
   bool ok = mn > 0.0f && fabsf(za - (2.0f / 3.0f)) >= c.eps_z;
   const float u = atan01(mn * __builtin_amdgcn_rcpf(mx)) * 0.63661977236758134f;  // quarter turns, [0, 0.5]
   const float w = ay > ax ? 1.0f - u : u;  // angle of (|x|, |y|) in quarter turns, [0, 1]
-  ok = ok && w >= c.eps_ax && w <= 1.0f - c.eps_ax;
-  // tt = phi / (pi/2) in [0, 4) = B + sw
-  const bool xneg = q.xf < 0.0f, yneg = q.yf < 0.0f;
+  ok = ok && fabsf(w - 0.5f) <= c.half_ax;
+  // tt = phi / (pi/2) in [0, 4) = B + sw; the signs from the sign bits (x, y are nonzero wherever ok holds)
+  const bool xneg = (int)__float_as_uint(q.xf) < 0, yneg = (int)__float_as_uint(q.yf) < 0;
   const int B = xneg ? 2 : (yneg ? 4 : 0);
   const float sw = xneg != yneg ? -w : w;
-  const float eps = c.eps_pix;
+  const float hp = c.half_pix;
   int pix;
   if (za < 2.0f / 3.0f) {
     // jp, jm = floor(N (0.5 + tt) -+ 0.75 N cth) = K0 + floor(h + N sw -+ 0.75 N cth), N (0.5 + B) = K0 + h
-    const int twoK = nside * (1 + 2 * B);
+    // products of operands below 2^24 (Nside <= 2^13) are 24-bit multiplies
+    const int twoK = (int)__umul24((unsigned)nside, (unsigned)(1 + 2 * B));
     const float h = (twoK & 1) ? 0.5f : 0.0f;
     const float nw = N * sw, nc = 0.75f * N * cth;
     const float v1 = (nw - nc) + h, v2 = (nw + nc) + h;
     const float f1 = floorf(v1), f2 = floorf(v2);
     const float d1 = v1 - f1, d2 = v2 - f2;
-    ok = ok && d1 >= eps && d1 <= 1.0f - eps && d2 >= eps && d2 <= 1.0f - eps;
+    ok = ok && fabsf(d1 - 0.5f) <= hp && fabsf(d2 - 0.5f) <= hp;
     const int jp = (twoK >> 1) + (int)f1, jm = (twoK >> 1) + (int)f2;
     const int ir = nside + 1 + jp - jm;
     const int kshift = 1 - (ir & 1);
-    int ip = (jp + jm - nside + kshift + 1) / 2;  // in [-Nside/2, 5 Nside + 1]
+    // (jp + jm - Nside + kshift + 1) / 2 in [-Nside/2, 5 Nside + 1]: halved with an even offset that makes it
+    // non-negative
+    int ip = (int)((unsigned)(jp + jm - nside + kshift + 1 + 8 * nside) >> 1) - 4 * nside;
     ip = ip < 0 ? ip + 4 * nside : (ip >= 4 * nside ? ip - 4 * nside : ip);
-    pix = nside * (nside - 1) * 2 + (ir - 1) * 4 * nside + ip;
+    pix = nside * (nside - 1) * 2 + (int)__umul24((unsigned)(ir - 1), (unsigned)(4 * nside)) + ip;
   }
   else {
     // polar caps: tp = tt - (int)tt, jp = floor(tp tmp), jm = floor((1 - tp) tmp), tmp = N sin(theta) sqrt(3/(1+za))
@@ -381,10 +388,12 @@ __device__ __forceinline__ int screen_pixel(const MapParams &m, const SampleS &q
     const float p3 = sw * (float)ir;
     const float g3 = floorf(p3);
     const float d3 = p3 - g3;
-    ok = ok && d1 >= eps && d1 <= 1.0f - eps && d2 >= eps && d2 <= 1.0f - eps && d3 >= eps && d3 <= 1.0f - eps;
-    int ip = B * ir + (int)g3;  // in [-ir, 5 ir]
+    ok = ok && fabsf(d1 - 0.5f) <= hp && fabsf(d2 - 0.5f) <= hp && fabsf(d3 - 0.5f) <= hp;
+    int ip = (int)__umul24((unsigned)B, (unsigned)ir) + (int)g3;  // in [-ir, 5 ir]
     ip = ip < 0 ? ip + 4 * ir : (ip >= 4 * ir ? ip - 4 * ir : ip);
-    pix = cth > 0.0f ? 2 * ir * (ir - 1) + ip : 12 * nside * nside - 2 * ir * (ir + 1) + ip;
+    const int tri = (int)__umul24((unsigned)ir, (unsigned)(ir - 1));  // ir (ir - 1); written 2 * tri, the
+                                                                      // product is no longer 24-bit
+    pix = cth > 0.0f ? tri + tri + ip : 12 * nside * nside - (tri + tri) - 4 * ir + ip;
   }
   return ok ? pix : -2;
 }
@@ -625,6 +634,8 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
       // the value of a sample in each window channel
       auto win_value = [&](int c) -> V { return value(mass_sub, (c >= 0 && c < m.n_nu) ? m.prefac[c] : 0.0); };
       const V v0 = win_value(cs.c0), v1 = win_value(cs.c0 + 1), v2 = win_value(cs.c0 + 2);
+      // the key of window slot k is kb + k npix (kb = c0 npix, modulo 2^bits(K) like every key)
+      const K kb = (K)((long long)cs.c0 * m.npix), kn1 = (K)m.npix, kn2 = (K)(2 * m.npix);
       const int wv = t >> 6;
       int qn = 0;  // entries in this wave's queue (the same in every lane)
       // the current run of samples with one key: its value is summed in registers, its slot claimed when it
@@ -632,13 +643,14 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
       PendingAdd<K, V> run;
       run.on = false;
       for (int j = 0; j < m.n_sub; j++) {
-        const int isub = ((const const_int *)m.sub_perm)[j];
+        const const_double *o = (const const_double *)m.sub_offp + 3 * j;
+        const double x = s.x0 + o[0], y = s.y0 + o[1], z = s.z0 + o[2];  // sample_pos's sums
         SampleS q;
-        screen_channel(m, s, cs, isub, q);
+        screen_channel_at(m, x, y, z, cs, q);
         const int p = q.inu >= 0 ? screen_pixel(m, q) : -1;
         const bool d = q.inu == -2 || p == -2;
         if (!d && q.inu >= 0 && (uint64_t)p < (uint64_t)m.npix) {
-          const K key = (K)((unsigned long long)(unsigned)q.inu * (unsigned)m.npix + (unsigned)p);
+          const K key = kb + (q.k == 2 ? kn2 : (q.k == 1 ? kn1 : (K)0)) + (K)p;
           const V v = q.k == 2 ? v2 : (q.k == 1 ? v1 : v0);
           if (run.on && key == run.key) {
             run.val += v;
@@ -652,8 +664,6 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
         if (md) {
           if (d) {
             const int e = qn + lane_rank(md);
-            double x, y, z;
-            sample_pos(m, s, isub, x, y, z);
             if (e < DEFER_CAP) {
               dq.x[wv][e] = x;
               dq.y[wv][e] = y;
