@@ -37,6 +37,8 @@ struct crime_gh {
   float2 *d_chwin = nullptr;   // [n_nu] decided interval of 1 + z of each channel (FP32 screen)
   void *d_amp = nullptr;       // amplitude per |k|^2
   uint32_t *d_order = nullptr; // map-stage tiles by distance from the observer
+  uint32_t *d_binstart = nullptr; // [nbins + 1] first tile of each distance bin in d_order
+  uint32_t *d_range = nullptr; // [2] the map kernel's range of d_order (k_tile_range)
   double *d_sub = nullptr;     // [3*n_sub]
   double *d_subp = nullptr;    // [3*n_sub] the offsets in the map stage's visiting order
   double *d_modes = nullptr;   // injected unit modes
@@ -44,6 +46,8 @@ struct crime_gh {
   double *d_partials = nullptr;
   double *d_sig = nullptr;     // [sigma2, mean, max |phi| of the grid (infinite when unknown)]
   double *h_sig = nullptr;     // pinned mirror
+  uint32_t *h_range = nullptr; // pinned mirror of d_range (the map stage's grid)
+  hipEvent_t ev_range = nullptr;
   int nparts = 0;
   FillParams fill{};
   MapParams map{};
@@ -320,13 +324,15 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_chwin, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_sub, h->d_subp,
+  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_chwin, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_binstart, h->d_range, h->d_sub, h->d_subp,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig, h->d_ps, h->d_ps_count, h->ps_maps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_sig) (void)hipHostFree(h->h_sig);
+  if (h->h_range) (void)hipHostFree(h->h_range);
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
+  if (h->ev_range) (void)hipEventDestroy(h->ev_range);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   return 0;
 }
@@ -387,6 +393,8 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
 
   HIP_OR_FAIL(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   for (auto &e : h->ev) HIP_OR_FAIL(h, hipEventCreate(&e));
+  HIP_OR_FAIL(h, hipEventCreateWithFlags(&h->ev_range, hipEventDisableTiming));
+  HIP_OR_FAIL(h, hipHostMalloc((void **)&h->h_range, 2 * sizeof(uint32_t), hipHostMallocDefault));
 
   HIP_OR_FAIL(h, hipMalloc(&h->F, h->ncell * 2 * esz));
   if (d->velocity_mode == CRIME_GH_VEL_SPECTRAL) {
@@ -629,6 +637,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     std::vector<uint32_t> count(nbins + 1, 0), order(ntiles);
     for (size_t b = 0; b < ntiles; b++) count[std::min(dist_bin(b), nbins - 1) + 1]++;
     for (size_t i = 1; i <= nbins; i++) count[i] += count[i - 1];
+    const std::vector<uint32_t> bin_start(count);  // count[b]: the first entry of bin b
     // each entry holds the tile's coordinates, tx | ty << 10 | tz << 20 (ntx, nty, ntz <= 512 for n <= 2048)
     for (size_t b = 0; b < ntiles; b++) {
       const uint32_t tx = (uint32_t)(b % m.ntx), ty = (uint32_t)((b / m.ntx) % m.nty),
@@ -638,6 +647,17 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     HIP_OR_FAIL(h, hipMalloc((void **)&h->d_order, ntiles * sizeof(uint32_t)));
     HIP_OR_FAIL(h, hipMemcpy(h->d_order, order.data(), ntiles * sizeof(uint32_t), hipMemcpyHostToDevice));
     m.tile_order = h->d_order;
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_binstart, (nbins + 1) * sizeof(uint32_t)));
+    HIP_OR_FAIL(h, hipMemcpy(h->d_binstart, bin_start.data(), (nbins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    m.bin_start = h->d_binstart;
+    m.nbins = (int)nbins;
+    // half-diagonal of a tile's box of cell centres, with a slack for the rounding of the bins' distances
+    const double hx = 0.5 * (TILE_X - 1), hy = 0.5 * (TILE_Y - 1), hz = 0.5 * (TILE_Z - 1);
+    m.tile_hd = dx * sqrt(hx * hx + hy * hy + hz * hz) + 1e-9 * dx * (double)nbins;
+    const uint32_t range0[2] = {0, (uint32_t)ntiles};
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_range, sizeof(range0)));
+    HIP_OR_FAIL(h, hipMemcpy(h->d_range, range0, sizeof(range0), hipMemcpyHostToDevice));
+    m.tile_range = h->d_range;
   }
   m.nside = h->nside;
   m.npix = h->npix;
@@ -698,16 +718,20 @@ static int enqueue_fields(crime_gh *h, bool timed, hipStream_t s = nullptr)
   return 0;
 }
 
+// s: a stream under capture (the workgroups then cover every tile); none: the handle's stream, whose map
+// launch is sized by the host from the tile range
 static int enqueue_maps(crime_gh *h, bool timed, hipStream_t s = nullptr)
 {
   hipError_t e;
+  const RangeReadback rb{h->h_range, h->ev_range};
+  const RangeReadback *prb = s ? nullptr : &rb;
   if (!s) s = h->stream;
   if (h->prec)
     e = launch_maps<double>((const cx<double> *)h->F, (double *)h->maps, h->nmaps, h->map, s,
-                            timed ? h->ev + 5 : nullptr, h->sched);
+                            timed ? h->ev + 5 : nullptr, h->sched, prb);
   else
     e = launch_maps<float>((const cx<float> *)h->F, (float *)h->maps, h->nmaps, h->map, s,
-                           timed ? h->ev + 5 : nullptr, h->sched);
+                           timed ? h->ev + 5 : nullptr, h->sched, prb);
   if (e != hipSuccess) return set_error(h, CRIME_GH_EHIP, "map kernels: %s", hipGetErrorString(e));
   return 0;
 }

@@ -176,6 +176,11 @@ struct MapParams {
   double dz_per_phi;           // |dz| <= dz_per_phi * sig[2] for every cell
   const uint32_t *tile_order;  // tiles by increasing distance of their centre from the observer, as
                                // tx | ty << 10 | tz << 20
+  const uint32_t *bin_start;   // [nbins + 1] first entry of tile_order in each distance bin (floor(d / dx) of
+                               // the tile's centre, the last bin also holding everything beyond); [nbins] = ntiles
+  int nbins;
+  double tile_hd;              // every cell centre of a tile lies within tile_hd of the tile's centre
+  uint32_t *tile_range;        // device [first, last): the entries of tile_order the map kernel visits
   int ntx, nty, ntz;
   int vmode;               // CRIME_GH_VEL_*: SPECTRAL reads vx from F.y and (vy, vz) from G
   const void *G;           // cx<T>[N^3] second grid of SPECTRAL handles (NULL otherwise)
@@ -234,9 +239,16 @@ hipError_t launch_power_spectrum(int n, cx<T> *F, const cx<T> *tw, double pscale
                                  double *acc, hipStream_t s);
 int fields_partials_count(int n, int prec);
 
+// The map stage's tile range to the host: with it, launch_maps waits (on the host, while the maps are
+// cleared) for the range k_tile_range found and launches exactly its workgroups; without it (a stream under
+// capture) the launch covers every tile and the workgroups past the range exit on their first load.
+struct RangeReadback {
+  uint32_t *h_range;   // pinned [2]
+  hipEvent_t ev;       // recorded after the copy of the range
+};
 template <typename T>
 hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m, hipStream_t s,
-                       hipEvent_t *ev, const Schedule &sc);
+                       hipEvent_t *ev, const Schedule &sc, const RangeReadback *rb = nullptr);
 template <typename T>
 hipError_t launch_cell_field(const cx<T> *F, const MapParams &m, int which, size_t cell0, size_t ncell, double *out,
                              hipStream_t s);

@@ -246,6 +246,20 @@ __device__ __forceinline__ bool tile_out_of_band(const MapParams &m, int tx, int
   return false;
 }
 
+// true when no tile of distance bin b can land a sample in a channel, with every cell's RSD shift within
+// dzb: the bin's tiles have their centres at [b dx, (b + 1) dx) from the observer (the last bin: beyond
+// b dx), so their cell centres at [b dx - tile_hd, (b + 1) dx + tile_hd]; widened by h_sub, the test of
+// tile_out_of_band on that range.  dzb infinite (or NaN): false.
+__device__ __forceinline__ bool bin_out_of_band(const MapParams &m, int b, double dzb)
+{
+  if (!(dzb < INFINITY)) return false;
+  const double rlo = b * m.dx - m.tile_hd, rhi = b + 1 >= m.nbins ? INFINITY : (b + 1) * m.dx + m.tile_hd;
+  const double zl = z_of_r(m.tab, rlo - m.h_sub) - dzb, zh = z_of_r(m.tab, rhi + m.h_sub) + dzb;
+  if (1 + zl > 0 && 1420.40575177 * (1 + 1e-10) < m.nu_lo * (1 + zl)) return true;
+  if (1 + zh > 0 && 1420.40575177 * (1 - 1e-10) >= m.nu_hi * (1 + zh)) return true;
+  return false;
+}
+
 // false when no sample of the cell can land in a channel (conservative); zm, zp as cell_z_range
 __device__ __forceinline__ bool cell_may_bin(const MapParams &m, const CellState &s, double zm, double zp)
 {
@@ -583,8 +597,9 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
   // tiles run in order of distance from the observer, so the workgroups resident at one time cover a
   // thin shell and their atomics fall on a few channels of the maps.  2-D grid: a dispatch holds at
   // most 2^32 work-items per dimension.
-  const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-  if (bid >= (size_t)m.ntx * m.nty * m.ntz) return;
+  // the workgroups take the entries [first, last) of the order that k_tile_range kept; the rest exit here
+  const size_t bid = (size_t)m.tile_range[0] + (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (bid >= (size_t)m.tile_range[1]) return;
   const uint32_t b = m.tile_order[bid];
   const int tx = (int)(b & 1023u), ty = (int)((b >> 10) & 1023u), tz = (int)(b >> 20);
   // whole tiles out of the band, by the grid's bound on the RSD shift: before any grid load
@@ -707,6 +722,31 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
   for (int i = t; i < SLOTS; i += 256) {
     const K k = keys[i];
     if (k != (K)~(K)0) global_add(vmaps + k, vals[i]);
+  }
+}
+
+// The range [first, last) of tile_order that the map kernel visits, for the current grid's RSD bound: from
+// the first entry of the nearest distance bin that bin_out_of_band keeps to the last entry of the farthest
+// (every tile when the bound is off).  One workgroup, before the map kernel.
+__global__ __launch_bounds__(256) void k_tile_range(MapParams m)
+{
+  __shared__ int lo, hi;
+  if (threadIdx.x == 0) {
+    lo = 0x7fffffff;
+    hi = -1;
+  }
+  __syncthreads();
+  const double dzb = m.precull ? m.dz_per_phi * m.sig[2] : INFINITY;
+  for (int b = threadIdx.x; b < m.nbins; b += 256)
+    if (m.bin_start[b] < m.bin_start[b + 1] && !bin_out_of_band(m, b, dzb)) {
+      atomicMin(&lo, b);
+      atomicMax(&hi, b);
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const bool any = lo <= hi;
+    m.tile_range[0] = any ? m.bin_start[lo] : 0u;
+    m.tile_range[1] = any ? m.bin_start[hi + 1] : 0u;
   }
 }
 
@@ -1189,9 +1229,10 @@ static inline unsigned nblk(size_t n)
 }
 
 template <typename T, bool FX>
-static void launch_pixelize(const cx<T> *F, T *maps, const MapParams &m, hipStream_t s, const Schedule &sc)
+static void launch_pixelize(const cx<T> *F, T *maps, const MapParams &m, hipStream_t s, const Schedule &sc,
+                            size_t ntiles)
 {
-  size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
+  if (ntiles == 0) return;
   unsigned gx = (unsigned)(ntiles < 65536 ? ntiles : 65536);
   unsigned gy = (unsigned)((ntiles + gx - 1) / gx);
   // 32-bit bin keys when every key and the empty marker fit
@@ -1228,9 +1269,15 @@ static void launch_pixelize(const cx<T> *F, T *maps, const MapParams &m, hipStre
 // ev[0] before clearing, ev[1] before the scatter, ev[2] after the last map kernel
 template <typename T>
 hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m, hipStream_t s,
-                       hipEvent_t *ev, const Schedule &sc)
+                       hipEvent_t *ev, const Schedule &sc, const RangeReadback *rb)
 {
   if (ev) (void)hipEventRecord(ev[0], s);
+  hipLaunchKernelGGL(k_tile_range, dim3(1), dim3(256), 0, s, m);
+  if (rb) {
+    hipError_t e = hipMemcpyAsync(rb->h_range, m.tile_range, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(rb->ev, s);
+    if (e != hipSuccess) return e;
+  }
   if (m.fx) {
     if (!m.fxmaps || !m.fxs || !m.mass_part) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(nblk(nmaps)), dim3(256), 0, s, m.fxmaps, nmaps);
@@ -1239,16 +1286,31 @@ hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m
     hipLaunchKernelGGL(k_zero<T>, dim3(nblk(nmaps)), dim3(256), 0, s, maps, nmaps);
   }
   if (ev) (void)hipEventRecord(ev[1], s);
+  // the workgroups: the range's size when the host has it (the clear above runs meanwhile), else every tile
+  size_t ntiles = (size_t)m.ntx * m.nty * m.ntz;
+  auto host_range = [&]() -> hipError_t {
+    const hipError_t e = hipEventSynchronize(rb->ev);
+    if (e == hipSuccess) ntiles = rb->h_range[1] > rb->h_range[0] ? rb->h_range[1] - rb->h_range[0] : 0;
+    return e;
+  };
   if (m.fx) {
     hipLaunchKernelGGL(k_mass_partials<T>, dim3(FX_PARTS), dim3(256), 0, s, F, m);
     hipLaunchKernelGGL(k_fx_scale, dim3(1), dim3(256), 0, s, m);
-    launch_pixelize<T, true>(F, maps, m, s, sc);
+    if (rb) {
+      const hipError_t e = host_range();
+      if (e != hipSuccess) return e;
+    }
+    launch_pixelize<T, true>(F, maps, m, s, sc, ntiles);
     int64_t b = (m.npix + 255) / 256;
     hipLaunchKernelGGL(k_fx_finalize<T>, dim3((unsigned)(b < 4096 ? b : 4096), m.n_nu), dim3(256), 0, s,
                        (const unsigned long long *)m.fxmaps, maps, m);
   }
   else {
-    launch_pixelize<T, false>(F, maps, m, s, sc);
+    if (rb) {
+      const hipError_t e = host_range();
+      if (e != hipSuccess) return e;
+    }
+    launch_pixelize<T, false>(F, maps, m, s, sc, ntiles);
   }
   if (ev) (void)hipEventRecord(ev[2], s);
   return hipGetLastError();
@@ -1337,7 +1399,7 @@ hipError_t launch_ps_expand(const double *A, const double *B, const double *c1, 
 
 #define CRIME_INST(T)                                                                                   \
   template hipError_t launch_maps<T>(const cx<T> *, T *, size_t, const MapParams &, hipStream_t,      \
-                                     hipEvent_t *, const Schedule &);                                   \
+                                     hipEvent_t *, const Schedule &, const RangeReadback *);            \
   template hipError_t launch_cell_field<T>(const cx<T> *, const MapParams &, int, size_t, size_t, double *,   \
                                            hipStream_t);                                                \
   template hipError_t launch_sample_indices<T>(const cx<T> *, const MapParams &, int64_t, int64_t,     \
