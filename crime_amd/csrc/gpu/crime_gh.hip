@@ -29,7 +29,7 @@ struct crime_gh {
   size_t ncell = 0;
   void *maps = nullptr;
   size_t nmaps = 0;
-  double *d_tab = nullptr;     // z_r2z | r_r2z | d_r2z | v_r2z
+  double *d_tab = nullptr;     // z_r2z | r_r2z | d_r2z | v_r2z | HI fraction | reference bias (at the nodes)
   double *d_pk = nullptr;      // logk | pk
   double *d_edges = nullptr;   // [n_nu+1]
   double *d_prefac = nullptr;  // [n_nu]
@@ -415,7 +415,7 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     HIP_OR_FAIL(h, hipMalloc((void **)&h->d_fx, h->nmaps * sizeof(unsigned long long)));
     HIP_OR_FAIL(h, hipMalloc((void **)&h->d_fxs, (2 + FX_PARTS) * sizeof(double)));
   }
-  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_tab, 4 * CRIME_GH_NZ * sizeof(double)));
+  HIP_OR_FAIL(h, hipMalloc((void **)&h->d_tab, 6 * CRIME_GH_NZ * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_pk, 2 * (size_t)d->numk * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_edges, (d->n_nu + 1) * sizeof(double)));
   HIP_OR_FAIL(h, hipMalloc((void **)&h->d_prefac, d->n_nu * sizeof(double)));
@@ -433,8 +433,14 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   }
 
   // tables
-  const double *tabs[4] = {d->z_r2z, d->r_r2z, d->d_r2z, d->v_r2z};
-  for (int i = 0; i < 4; i++)
+  // the HI fraction (src/user_defined.c:27-30) and the reference bias (src/user_defined.c:32-35) at the nodes
+  std::vector<double> fhi(CRIME_GH_NZ), bref(CRIME_GH_NZ);
+  for (int i = 0; i < CRIME_GH_NZ; i++) {
+    fhi[i] = 0.008 * pow(1 + d->z_r2z[i], 0.6);
+    bref[i] = 0.904 + 0.135 * pow(1 + d->z_r2z[i], 1.696);
+  }
+  const double *tabs[6] = {d->z_r2z, d->r_r2z, d->d_r2z, d->v_r2z, fhi.data(), bref.data()};
+  for (int i = 0; i < 6; i++)
     HIP_OR_FAIL(h, hipMemcpy(h->d_tab + (size_t)i * CRIME_GH_NZ, tabs[i], CRIME_GH_NZ * sizeof(double),
                              hipMemcpyHostToDevice));
   HIP_OR_FAIL(h, hipMemcpy(h->d_pk, d->logk, d->numk * sizeof(double), hipMemcpyHostToDevice));
@@ -578,11 +584,15 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
   m.tab.r = h->d_tab + CRIME_GH_NZ;
   m.tab.d = h->d_tab + 2 * CRIME_GH_NZ;
   m.tab.v = h->d_tab + 3 * CRIME_GH_NZ;
+  m.tab.f = h->d_tab + 4 * CRIME_GH_NZ;
+  m.tab.b = h->d_tab + 5 * CRIME_GH_NZ;
   m.tab.idr = d->idr;
   m.tab.r_last = d->r_r2z[CRIME_GH_NZ - 1];
   m.tab.z_last = d->z_r2z[CRIME_GH_NZ - 1];
   m.tab.d_last = d->d_r2z[CRIME_GH_NZ - 1];
   m.tab.v_last = d->v_r2z[CRIME_GH_NZ - 1];
+  m.tab.f_last = 0.008 * pow(1 + m.tab.z_last, 0.6);
+  m.tab.b_last = 0.904 + 0.135 * pow(1 + m.tab.z_last, 1.696);
   m.bias_model = d->bias_model;
   m.b0 = d->bias_b0;
   m.b1 = d->bias_b1;

@@ -82,9 +82,11 @@ struct RTables {
   const double *r;    // r_arr_r2z
   const double *d;    // growth_d_arr
   const double *v;    // growth_v_arr
+  const double *f;    // 0.008 (1 + z)^0.6 at the nodes (the HI fraction of FP32 grids' map stage)
+  const double *b;    // 0.904 + 0.135 (1 + z)^1.696 at the nodes (the reference bias, likewise)
   double idr;
   double r_last;      // r_arr_r2z[NZ-1]
-  double z_last, d_last, v_last;
+  double z_last, d_last, v_last, f_last, b_last;
 };
 
 // Branch-free: the node index is clamped into the table for every r (NaN gives node 0), so the three
@@ -111,6 +113,16 @@ __device__ __forceinline__ double dgrowth_of_r(const RTables &t, double r)
 __device__ __forceinline__ double vgrowth_of_r(const RTables &t, double r)
 {
   return tab_lerp(t.v, t.r, t.idr, r, t.r_last, t.v_last, 1.0);
+}
+// the node tables of the HI fraction and of the reference bias, interpolated like z(r): within 1e-8
+// (relative) of the formulas at z(r) for tables of the reference's resolution
+__device__ __forceinline__ double fhi_of_r(const RTables &t, double r)
+{
+  return tab_lerp(t.f, t.r, t.idr, r, t.r_last, t.f_last, 0.008);
+}
+__device__ __forceinline__ double bref_of_r(const RTables &t, double r)
+{
+  return tab_lerp(t.b, t.r, t.idr, r, t.r_last, t.b_last, 0.904 + 0.135);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -83,32 +83,51 @@ __device__ __forceinline__ void cell_velocity(const MapParams &m, const cx<T> *_
 // lognormal HI mass and RSD shift (grid_tools.c:138-146)
 // the radial tables at the cell centre, split from the arithmetic so that a kernel can issue their loads
 // with the grid's: z, D and V of r
+// NODES (the map stage of FP32 grids): the HI fraction and the reference bias from their node tables
+// (within 1e-8 relative of the formulas, far below FP32 map precision) instead of two FP64 pow per cell;
+// the FP64 grids' map stage and every parity kernel evaluate the formulas.
 struct CellTabs {
-  double z, d, v;
+  double z, d, v, f, b;
 };
+template <bool NODES = false>
 __device__ __forceinline__ void cell_tabs(const MapParams &m, const CellState &s, CellTabs &t)
 {
   t.z = z_of_r(m.tab, s.r);
   t.d = dgrowth_of_r(m.tab, s.r);
   t.v = vgrowth_of_r(m.tab, s.r);
+  if constexpr (NODES) {
+    t.f = fhi_of_r(m.tab, s.r);
+    t.b = m.bias_model == 0 ? bref_of_r(m.tab, s.r) : 0.0;
+  }
 }
 
+template <bool NODES = false>
 __device__ __forceinline__ void cell_hi_tabs(const MapParams &m, double delta, double sigma2, const CellTabs &t,
                                              CellState &s)
 {
   const double redshift = t.z;
   s.zc = redshift;
-  double gfd = t.d * bias_of(m, redshift);
+  double bias, fhi;
+  if constexpr (NODES) {
+    bias = m.bias_model == 0 ? t.b : bias_of(m, redshift);
+    fhi = t.f;
+  }
+  else {
+    bias = bias_of(m, redshift);
+    fhi = fraction_hi(redshift);
+  }
+  double gfd = t.d * bias;
   s.dz = m.rsd ? s.vr * t.v : 0.0;
   double dens_ln = exp(gfd * (delta - 0.5 * gfd * sigma2));
-  s.mass = m.mass_prefac * fraction_hi(redshift) * dens_ln;
+  s.mass = m.mass_prefac * fhi * dens_ln;
 }
 
+template <bool NODES = false>
 __device__ __forceinline__ void cell_hi(const MapParams &m, double delta, double sigma2, CellState &s)
 {
   CellTabs t;
-  cell_tabs(m, s, t);
-  cell_hi_tabs(m, delta, sigma2, t, s);
+  cell_tabs<NODES>(m, s, t);
+  cell_hi_tabs<NODES>(m, delta, sigma2, t, s);
 }
 
 // The sample offsets are the same for every lane and never written by a kernel: read through the
@@ -616,7 +635,7 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
     const size_t cell = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
     delta = (double)F[cell].x;
     cell_velocity<T>(m, F, ix, iy, iz, s);
-    cell_tabs(m, s, tb);
+    cell_tabs<sizeof(T) == 4>(m, s, tb);
     cell_z_range(m, s, zm, zp);
     s.dz = m.rsd ? s.vr * tb.v : 0.0;
     active = cell_may_bin(m, s, zm, zp);
@@ -628,7 +647,7 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
   }
   __syncthreads();
   if (active) {
-    cell_hi_tabs(m, delta, m.sig[0], tb, s);
+    cell_hi_tabs<sizeof(T) == 4>(m, delta, m.sig[0], tb, s);
     double mass_sub = s.mass / m.n_sub;
     // the value a sample adds: its mK (FP) or its fixed-point mass (FX), from the cell's mass per sample
     auto value = [&](double w, double pf) -> V {
@@ -765,7 +784,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
   pixelize_body<T, SLOTS, SEG, FX, K>(F, maps, m);
 }
 
-// Deterministic scatter, step 1: the box's HI mass (the same cell_hi as the map stage) summed in a fixed
+// Deterministic scatter, step 1: the box's HI mass (the same cell_hi as the map kernel) summed in a fixed
 // order -- a fixed grid of FX_PARTS blocks, each thread over a fixed stride of cells, then a tree in LDS.
 template <typename T>
 __global__ __launch_bounds__(256) void k_mass_partials(const cx<T> *__restrict__ F, MapParams m)
@@ -782,7 +801,7 @@ __global__ __launch_bounds__(256) void k_mass_partials(const cx<T> *__restrict__
     s.z0 = m.dx * (iz + 0.5) - m.obs[2];
     s.r = sqrt(s.x0 * s.x0 + s.y0 * s.y0 + s.z0 * s.z0);
     s.vr = 0;
-    cell_hi(m, (double)F[cell].x, m.sig[0], s);
+    cell_hi<sizeof(T) == 4>(m, (double)F[cell].x, m.sig[0], s);
     acc += s.mass;
   }
   red[threadIdx.x] = acc;
