@@ -225,17 +225,30 @@ def fold_standardisation(p, x_mean, x_std, y_mean, y_std):
 
 
 class Emulator:
-    """A trained Mlp whose parameters have the input and output standardisation folded in; targets
-    y = asinh(C / scale)."""
+    """Trained Mlps (an ensemble of networks that differ in their initial weights and batch draws) whose
+    parameters have the input and output standardisation folded in; targets y = asinh(C / scale), the
+    prediction the networks' mean y.  One network's error on held-out cosmologies depends on where its
+    training ends as much as on the data (a change of 1e-8 in the targets can move it several-fold); the
+    mean of a few is steadier and smaller."""
 
-    def __init__(self, mlp, x_mean, x_std, y_mean, y_std, scale, dnu):
-        self.mlp, self.scale, self.dnu = mlp, scale, dnu
-        mlp.set_params(fold_standardisation(mlp.params(), x_mean, x_std, y_mean, y_std))
+    def __init__(self, mlps, x_mean, x_std, y_mean, y_std, scale, dnu):
+        self.mlps = list(mlps) if isinstance(mlps, (list, tuple)) else [mlps]
+        self.mlp = self.mlps[0]
+        self.scale, self.dnu = scale, dnu
+        for mlp in self.mlps:
+            mlp.set_params(fold_standardisation(mlp.params(), x_mean, x_std, y_mean, y_std))
 
     def predict_y(self, x):
         z = np.array(x, dtype=np.float32, copy=True)
         z[:, 6] = (z[:, 6] - z[:, 5]) * np.float32(1.0 / self.dnu)
-        return self.mlp.forward(z)
+        y = self.mlps[0].forward(z)
+        for mlp in self.mlps[1:]:
+            y = y + mlp.forward(z)
+        return y / np.float32(len(self.mlps)) if len(self.mlps) > 1 else y
+
+    def close(self):
+        for mlp in self.mlps:
+            mlp.close()
 
     def predict(self, x):
         """Binned C_l at raw inputs x (n, 8)."""
@@ -255,9 +268,10 @@ def gaussian_sigma(mean, spec, nseeds):
     return sig
 
 
-def train_emulator(cosmos, cls, spec, steps=4000, batch=4096, lr0=2e-3, seed=0, log=None):
+def train_emulator(cosmos, cls, spec, steps=4000, batch=4096, lr0=2e-3, seed=0, log=None, n_nets=1):
     """Fit asinh(C / scale) of the seed-averaged binned C_l by chi^2 (weights 1/sigma_y^2 from the
-    Gaussian cosmic variance of each target).  Returns (Emulator, per-step losses)."""
+    Gaussian cosmic variance of each target), with n_nets networks (seeds seed, seed + 1, ...).  Returns
+    (Emulator, per-step losses averaged over the networks)."""
     dnu = (spec.nu_hi - spec.nu_lo) / spec.n_nu
     x = encode(features(cosmos, spec), dnu)
     mean = cls.mean(axis=1)
@@ -271,13 +285,17 @@ def train_emulator(cosmos, cls, spec, steps=4000, batch=4096, lr0=2e-3, seed=0, 
     xm, xs = x.mean(axis=0), x.std(axis=0)
     xs[xs == 0] = 1.0
     ym, ys = y.mean(), y.std()
-    mlp = Mlp(max_batch=max(batch, 65536))
-    mlp.set_params(init_params(seed))
-    mlp.set_data((x - xm) / xs, (y - ym) / ys, w)
-    losses = mlp.train(steps, batch, lr0, steps, seed=seed)
-    if log:
-        log(f"trained {steps} steps: loss {losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}")
-    return Emulator(mlp, xm, xs, ym, ys, scale, dnu), losses
+    mlps, losses = [], []
+    for k in range(n_nets):
+        mlp = Mlp(max_batch=max(batch, 65536))
+        mlp.set_params(init_params(seed + k))
+        mlp.set_data((x - xm) / xs, (y - ym) / ys, w)
+        losses.append(mlp.train(steps, batch, lr0, steps, seed=seed + k))
+        mlps.append(mlp)
+        if log:
+            log(f"network {k + 1}/{n_nets}: {steps} steps, loss {losses[-1][:10].mean():.3e} -> "
+                f"{losses[-1][-50:].mean():.3e}")
+    return Emulator(mlps, xm, xs, ym, ys, scale, dnu), np.mean(losses, axis=0)
 
 
 def init_params(seed):
