@@ -560,15 +560,17 @@ __device__ __forceinline__ void agg_finish(PendingAdd<K, T> &pa, K *keys, T *val
   else agg_add<T, SLOTS, K>(pa.key, (pa.h + 1) & (SLOTS - 1), pa.val, keys, vals, maps, 63);
 }
 
-// Samples whose FP32 decision missed a margin, queued per wave in LDS and resolved by the FP64 path
-// together by all lanes of the wave after the last sample, so that the FP64 path runs about once per
-// wave instead of in every sample step in which one lane needs it (with 128 samples per wave and step,
-// that is most steps).  A sample that finds the queue full is resolved in its step.  dz is the cell's
-// RSD shift, w its mass per sample, inu the channel or -2 when the channel is undecided.
-constexpr int DEFER_CAP = 32;  // entries per wave
+// Samples whose FP32 decision missed a margin, queued per workgroup in LDS and resolved by the FP64
+// path after the workgroup's last sample, one entry per thread: a tile's few entries (about a dozen at
+// 1024^3) then cost one wave's pass through the FP64 path instead of one pass per wave that queued any.
+// A wave's step claims its entries with one LDS atomic; a sample that finds the queue full is resolved in
+// its step.  dz is the cell's RSD shift, w its mass per sample, inu the channel or -2 when the channel is
+// undecided.  n counts the claims, full or not.
+constexpr int DEFER_CAP = 128;  // entries per workgroup
 struct DeferQueue {
-  double x[4][DEFER_CAP], y[4][DEFER_CAP], z[4][DEFER_CAP], dz[4][DEFER_CAP], w[4][DEFER_CAP];
-  int inu[4][DEFER_CAP];
+  double x[DEFER_CAP], y[DEFER_CAP], z[DEFER_CAP], dz[DEFER_CAP], w[DEFER_CAP];
+  int inu[DEFER_CAP];
+  int n;
 };
 
 // rank of this lane among the lanes set in mask below it
@@ -645,22 +647,23 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
     keys[i] = (K)~(K)0;
     vals[i] = (V)0;
   }
+  if (t == 0) dq.n = 0;
   __syncthreads();
+  // the value a sample adds: its mK (FP) or its fixed-point mass (FX), from the cell's mass per sample
+  auto value = [&](double w, double pf) -> V {
+    if constexpr (FX) return (V)__double2ull_rn(w * m.fxs[0]);
+    else return (V)(w * pf);
+  };
+  // npix = 12 Nside^2 < 2^32: the key is one 32 x 32 -> 64-bit multiply-add
+  auto add_v = [&](int inu, int64_t ipix, V v) {
+    if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix) {
+      const K key = (K)((unsigned long long)(unsigned)inu * (unsigned)m.npix + (unsigned)ipix);
+      agg_add<V, SLOTS, K>(key, home_slot<SLOTS, SEG, K>(key), v, keys, vals, vmaps);
+    }
+  };
   if (active) {
     cell_hi_tabs<sizeof(T) == 4>(m, delta, m.sig[0], tb, s);
     double mass_sub = s.mass / m.n_sub;
-    // the value a sample adds: its mK (FP) or its fixed-point mass (FX), from the cell's mass per sample
-    auto value = [&](double w, double pf) -> V {
-      if constexpr (FX) return (V)__double2ull_rn(w * m.fxs[0]);
-      else return (V)(w * pf);
-    };
-    // npix = 12 Nside^2 < 2^32: the key is one 32 x 32 -> 64-bit multiply-add
-    auto add_v = [&](int inu, int64_t ipix, V v) {
-      if (inu >= 0 && (uint64_t)ipix < (uint64_t)m.npix) {
-        const K key = (K)((unsigned long long)(unsigned)inu * (unsigned)m.npix + (unsigned)ipix);
-        agg_add<V, SLOTS, K>(key, home_slot<SLOTS, SEG, K>(key), v, keys, vals, vmaps);
-      }
-    };
     auto add = [&](int inu, int64_t ipix, double pf) { add_v(inu, ipix, value(mass_sub, pf)); };
     if (m.nu_mode == 0 && m.scr.on) {
       CellScreen cs;
@@ -670,8 +673,6 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
       const V v0 = win_value(cs.c0), v1 = win_value(cs.c0 + 1), v2 = win_value(cs.c0 + 2);
       // the key of window slot k is kb + k npix (kb = c0 npix, modulo 2^bits(K) like every key)
       const K kb = (K)((long long)cs.c0 * m.npix), kn1 = (K)m.npix, kn2 = (K)(2 * m.npix);
-      const int wv = t >> 6;
-      int qn = 0;  // entries in this wave's queue (the same in every lane)
       // the current run of samples with one key: its value is summed in registers, its slot claimed when it
       // starts and its sum added when it ends
       PendingAdd<K, V> run;
@@ -696,15 +697,20 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
         }
         const unsigned long long md = __ballot(d);
         if (md) {
+          // the step's entries, claimed by the first lane that has one
+          const int lead = __builtin_ctzll(md);
+          int base = 0;
+          if ((t & 63) == lead) base = atomicAdd(&dq.n, __popcll(md));
+          base = __builtin_amdgcn_readlane(base, lead);
           if (d) {
-            const int e = qn + lane_rank(md);
+            const int e = base + lane_rank(md);
             if (e < DEFER_CAP) {
-              dq.x[wv][e] = x;
-              dq.y[wv][e] = y;
-              dq.z[wv][e] = z;
-              dq.dz[wv][e] = s.dz;
-              dq.w[wv][e] = mass_sub;
-              dq.inu[wv][e] = q.inu;
+              dq.x[e] = x;
+              dq.y[e] = y;
+              dq.z[e] = z;
+              dq.dz[e] = s.dz;
+              dq.w[e] = mass_sub;
+              dq.inu[e] = q.inu;
             }
             else {  // queue full: the FP64 decision in this step
               int64_t ipix;
@@ -713,20 +719,9 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
               add(c, ipix, pf);
             }
           }
-          qn = min(qn + __popcll(md), DEFER_CAP);
         }
       }
       agg_finish<V, SLOTS, K>(run, keys, vals, vmaps);
-      // resolve the queue: entry e by the e-th active lane (mod the number of active lanes)
-      const unsigned long long ex = __ballot(1);
-      const int nact = __popcll(ex), rank = lane_rank(ex);
-      for (int e = rank; e < qn; e += nact) {
-        int64_t ipix;
-        double pf;
-        const int inu = resolve_deferred(m, dq.x[wv][e], dq.y[wv][e], dq.z[wv][e], dq.dz[wv][e], dq.inu[wv][e],
-                                         ipix, pf);
-        if (inu >= 0) add_v(inu, ipix, value(dq.w[wv][e], pf));
-      }
     }
     else {
       for (int isub = 0; isub < m.n_sub; isub++) {
@@ -736,6 +731,15 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
         add(inu, ipix, pf);
       }
     }
+  }
+  __syncthreads();
+  // the queued samples, one per thread (the first wave only, for up to 64 entries)
+  const int nq = min(dq.n, DEFER_CAP);
+  for (int e = t; e < nq; e += 256) {
+    int64_t ipix;
+    double pf;
+    const int inu = resolve_deferred(m, dq.x[e], dq.y[e], dq.z[e], dq.dz[e], dq.inu[e], ipix, pf);
+    if (inu >= 0) add_v(inu, ipix, value(dq.w[e], pf));
   }
   __syncthreads();
   for (int i = t; i < SLOTS; i += 256) {
