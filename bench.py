@@ -25,8 +25,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # VALU issue peak of the MI355X: its published FP32 vector peak, 157.3 TFLOP/s, is 7.86e13 lane-FMAs per
-# second = 256 CUs x 4 SIMDs x 32 lanes x 2.4 GHz (a wave64 FP32 instruction every 2 cycles per SIMD; FP64
-# and transcendental instructions take longer, so a mixed kernel stays below it)
+# second = 256 CUs x 4 SIMDs x 32 lanes x 2.4 GHz, a wave64 instruction every 2 cycles per SIMD.  Measured on
+# the MI355X, only the simplest instructions issue that fast (FP32 add/mul, 32-bit add/and, moves); FP32 fma,
+# compares, selects, floors and conversions take 4 cycles, FP64 arithmetic and 32-bit multiplies 4.8,
+# v_rcp/v_sqrt/v_rsq 8 -- so this is an upper bound, and the roofline also reports the VALU busy share of
+# the kernel's actual instruction mix
 VALU_PEAK_LANE_INSTR = 256 * 4 * 32 * 2.4e9
 WORKLOADS = ("production_1024", "small_fp64_256", "largest_2048")
 METRIC = {"production_1024": "cells/s end-to-end at 1024^3",
@@ -492,6 +495,8 @@ def main():
         ms_pix = launch["pixelize"]
         atomics = profile_values(args.workload, "global_atomics").get("k_pixelize")
         valu = profile_values(args.workload, "valu_lane_instr").get("k_pixelize")
+        vcyc = (profile_values(args.workload, "valu_cycles_lo").get("k_pixelize"),
+                profile_values(args.workload, "valu_cycles_hi").get("k_pixelize"))
         pixelize = {"ms_per_launch": ms_pix, "samples": cells * d.n_sub, "binned_samples": n_binned,
                     "samples_per_s": cells * d.n_sub / (ms_pix * 1e-3), "binned_samples_per_s": n_binned / (ms_pix * 1e-3),
                     "global_atomics_per_launch": atomics,
@@ -503,6 +508,9 @@ def main():
                     "frac": valu / (ms_pix * 1e-3) / VALU_PEAK_LANE_INSTR,
                     "traffic": measured_traffic(args.workload).get("k_pixelize"),
                     "valu_lane_instr_per_launch": valu, "ms_per_launch": ms_pix,
+                    # VALU issue cycles per launch from the committed instruction-mix profile and the measured
+                    # per-instruction costs, as a share of the 1024 SIMDs' cycles (lower, upper bound)
+                    "valu_busy": ([v / (1024 * 2.4e9 * ms_pix * 1e-3) for v in vcyc] if all(vcyc) else None),
                     "hbm": {"bytes_per_launch": by["pixelize"], "achieved_GB/s": achieved if dom == "pixelize" else None,
                             "frac": (achieved / HBM_PEAK_GBS) if dom == "pixelize" else None},
                     "note": "VALU lane-instructions per launch from the committed SQ_INSTS_VALU profile x 64, "
