@@ -257,6 +257,8 @@ def gpu_lib():
         lib.crime_mlp_get_params.argtypes = [H, _fp]
         lib.crime_mlp_forward.restype = C.c_int
         lib.crime_mlp_forward.argtypes = [H, C.c_int64, _fp, _fp]
+        lib.crime_mlp_forward_mean.restype = C.c_int
+        lib.crime_mlp_forward_mean.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int64, _fp, _fp]
         lib.crime_mlp_loss_grad.restype = C.c_int
         lib.crime_mlp_loss_grad.argtypes = [H, C.c_int, _fp, _fp, _fp, _dp, _fp]
         lib.crime_mlp_set_data.restype = C.c_int

@@ -465,18 +465,21 @@ extern "C" int crime_mlp_get_params(crime_mlp_t *h, float *params)
 }
 
 // forward over rows [0, n) of h->X
-static void enqueue_forward(crime_mlp *h, int n, int train)
+// forward for rows [0, n) of X (the handle's own input buffer unless given) on stream s (its own unless given)
+static void enqueue_forward(crime_mlp *h, int n, int train, hipStream_t s = nullptr, const float *X = nullptr)
 {
   const size_t B = (size_t)h->max_batch;
-  hipLaunchKernelGGL(k_in_fwd, dim3(nblk((int64_t)n * WD)), dim3(256), 0, h->stream, h->X, h->p, n, h->Z, h->H);
+  if (!s) s = h->stream;
+  if (!X) X = h->X;
+  hipLaunchKernelGGL(k_in_fwd, dim3(nblk((int64_t)n * WD)), dim3(256), 0, s, X, h->p, n, h->Z, h->H);
   const dim3 g((n + TM - 1) / TM, WD / TN, 1);
   for (int k = 0; k < NMID; k++) {
     EpiFwd e{h->p + OFF_MID + k * MID_STRIDE + (int64_t)WD * WD, h->Z + (size_t)(k + 1) * B * WD,
              h->H + (size_t)(k + 1) * B * WD};
-    hipLaunchKernelGGL((k_gemm<false, false, EpiFwd>), g, dim3(256), 0, h->stream, h->H + (size_t)k * B * WD, WD,
+    hipLaunchKernelGGL((k_gemm<false, false, EpiFwd>), g, dim3(256), 0, s, h->H + (size_t)k * B * WD, WD,
                        h->wbf + (size_t)k * WD * WD, WD, n, WD, WD, e);
   }
-  hipLaunchKernelGGL(k_out, dim3((n + 3) / 4), dim3(256), 0, h->stream, h->H + (size_t)(NH - 1) * B * WD,
+  hipLaunchKernelGGL(k_out, dim3((n + 3) / 4), dim3(256), 0, s, h->H + (size_t)(NH - 1) * B * WD,
                      h->Z + (size_t)(NH - 1) * B * WD, h->p, h->T, h->Wt, n, train, h->Y, h->dy,
                      h->dZ + (size_t)(NH - 1) * B * WD, h->st, h->loss);
 }
@@ -560,6 +563,49 @@ extern "C" int crime_mlp_forward(crime_mlp_t *h, int64_t n, const float *x, floa
     MLP_HIP(h, hipMemcpyAsync(y + i0, h->Y, (size_t)c * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   }
   MLP_HIP(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// acc[i] (+)= w y[i]: the ensemble mean of crime_mlp_forward_mean
+__global__ __launch_bounds__(256) void k_acc_rows(float *__restrict__ acc, const float *__restrict__ y, int n, float w,
+                                                  int first)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) acc[i] = first ? w * y[i] : acc[i] + w * y[i];
+}
+
+extern "C" int crime_mlp_forward_mean(crime_mlp_t *const *hs, int nh, int64_t n, const float *x, float *y)
+{
+  if (!hs || nh < 1 || !hs[0]) return mlp_error(nullptr, CRIME_GH_EINVAL, "need nh >= 1 handles");
+  crime_mlp *h0 = hs[0];
+  if (n < 1 || !x || !y) return mlp_error(h0, CRIME_GH_EINVAL, "need n >= 1 and non-NULL x, y");
+  int batch = h0->max_batch;
+  for (int k = 0; k < nh; k++) {
+    if (!hs[k]) return mlp_error(h0, CRIME_GH_EINVAL, "handle %d is NULL", k);
+    if (hs[k]->device != h0->device) return mlp_error(h0, CRIME_GH_EINVAL, "handle %d is on another device", k);
+    batch = std::min(batch, hs[k]->max_batch);
+  }
+  MLP_HIP(h0, hipSetDevice(h0->device));
+  for (int k = 0; k < nh; k++) {
+    int rc = ensure_loss(hs[k], 1);
+    if (rc) return rc;
+    MLP_HIP(h0, hipStreamSynchronize(hs[k]->stream));  // no pending work of a handle on its buffers
+  }
+  // every network reads the first handle's input buffer on the first handle's stream; the mean collects in
+  // its target buffer (scratch outside a training step)
+  hipStream_t s = h0->stream;
+  const float w = 1.0f / (float)nh;
+  for (int64_t i0 = 0; i0 < n; i0 += batch) {
+    const int c = (int)(n - i0 < batch ? n - i0 : batch);
+    MLP_HIP(h0, hipMemcpyAsync(h0->X, x + i0 * NIN, (size_t)c * NIN * sizeof(float), hipMemcpyHostToDevice, s));
+    for (int k = 0; k < nh; k++) {
+      enqueue_forward(hs[k], c, 0, s, h0->X);
+      hipLaunchKernelGGL(k_acc_rows, dim3((c + 255) / 256), dim3(256), 0, s, h0->T, hs[k]->Y, c, w, k == 0);
+    }
+    MLP_HIP(h0, hipGetLastError());
+    MLP_HIP(h0, hipMemcpyAsync(y + i0, h0->T, (size_t)c * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
+  MLP_HIP(h0, hipStreamSynchronize(s));
   return 0;
 }
 

@@ -67,6 +67,15 @@ class Mlp:
         self._check(self._lib.crime_mlp_forward(self._h, x.shape[0], _fp(x), _fp(y)))
         return y
 
+    @staticmethod
+    def forward_mean(mlps, x):
+        """The mean of the networks' outputs at x, the inputs sent to the device once (crime_mlp_forward_mean)."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NIN)
+        y = np.empty(x.shape[0], np.float32)
+        hs = (C.c_void_p * len(mlps))(*[m._h.value for m in mlps])
+        mlps[0]._check(mlps[0]._lib.crime_mlp_forward_mean(hs, len(mlps), x.shape[0], _fp(x), _fp(y)))
+        return y
+
     def loss_grad(self, x, t, w=None):
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NIN)
         t = np.ascontiguousarray(t, dtype=np.float32)
@@ -241,10 +250,7 @@ class Emulator:
     def predict_y(self, x):
         z = np.array(x, dtype=np.float32, copy=True)
         z[:, 6] = (z[:, 6] - z[:, 5]) * np.float32(1.0 / self.dnu)
-        y = self.mlps[0].forward(z)
-        for mlp in self.mlps[1:]:
-            y = y + mlp.forward(z)
-        return y / np.float32(len(self.mlps)) if len(self.mlps) > 1 else y
+        return Mlp.forward_mean(self.mlps, z) if len(self.mlps) > 1 else self.mlps[0].forward(z)
 
     def close(self):
         for mlp in self.mlps:

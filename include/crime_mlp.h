@@ -48,6 +48,11 @@ int crime_mlp_get_params(crime_mlp_t *h, float *params);
 /* y[i] for host inputs x[i][8], any n >= 1 (run in batches of max_batch). */
 int crime_mlp_forward(crime_mlp_t *h, int64_t n, const float *x, float *y);
 
+/* The mean of nh networks' y[i] (an ensemble on one device): each batch of inputs goes to the device once
+ * and every network runs on it, on the first handle's stream; batches of the smallest max_batch.  The
+ * handles' own streams are synchronised first.  No counterpart in the reference (the emulator's). */
+int crime_mlp_forward_mean(crime_mlp_t *const *hs, int nh, int64_t n, const float *x, float *y);
+
 /* Loss and its gradient (FP32, parameter order above) for one host batch, n <= max_batch, n % 64 == 0;
  * w may be NULL (unit weights).  The parameters are not changed. */
 int crime_mlp_loss_grad(crime_mlp_t *h, int n, const float *x, const float *t, const float *w, double *loss,

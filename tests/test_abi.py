@@ -126,5 +126,6 @@ def test_mlp_abi_without_device():
     assert h.value is None
     assert b"max_batch" in lib.crime_mlp_last_error(None)
     assert lib.crime_mlp_forward(None, 1, None, None) == -1
+    assert lib.crime_mlp_forward_mean(None, 1, 1, None, None) == -1
     assert lib.crime_mlp_train(None, 1, 64, 1e-3, 10, 0.9, 0.999, 1e-8, 0, None) == -1
     lib.crime_mlp_destroy(None)

@@ -40,6 +40,23 @@ def test_forward_vs_oracle():
     assert q[0] < 1e-6 and q[1] < 5e-3 and q[2] < 5e-2
 
 
+def test_forward_mean_of_networks():
+    """crime_mlp_forward_mean (inputs sent once, networks of different max_batch) against the mean of each
+    network's own forward: the same rows in the same kernels, so equal up to the FP32 rounding of the mean."""
+    x, _ = data(3000, 5)   # several batches of the smaller max_batch, the last one partial
+    nets = [Mlp(max_batch=1024), Mlp(max_batch=512), Mlp(max_batch=1024)]
+    try:
+        for k, net in enumerate(nets):
+            net.set_params(mo.init_params(10 + k))
+        each = np.mean([net.forward(x) for net in nets], axis=0)
+        mean = Mlp.forward_mean(nets, x)
+        assert np.allclose(mean, each, rtol=1e-6, atol=1e-6 * np.abs(each).max())
+        assert np.array_equal(Mlp.forward_mean(nets[:1], x), nets[0].forward(x))
+    finally:
+        for net in nets:
+            net.close()
+
+
 def test_loss_and_gradient_vs_oracle():
     p = mo.init_params(3)
     x, t = data(1024, 4)
