@@ -600,7 +600,7 @@ __device__ __forceinline__ int resolve_deferred(const MapParams &m, double x, do
 // global atomics scale with the bins a tile touches instead of with its samples.  SLOTS (2048, 1024 or
 // 4096 bins) trades occupancy against probe length and overflow; SEG groups the bins of 2^SEG adjacent
 // map words (home_slot); K is the bin key type (32-bit when n_nu npix < 2^32 - 1, which halves the key
-// table).  In TABLE mode the FP32 screen decides the samples and its FP64 fallbacks are queued per wave.
+// table).  In TABLE mode the FP32 screen decides the samples and its FP64 fallbacks are queued per workgroup.
 // FX (deterministic scatter): every sample adds the integer round(s mass / n_sub) to the 64-bit bin of
 // m.fxmaps instead of its FP mK value to maps; k_fx_finalize applies 1/s and the channel factor.
 template <typename T, int SLOTS, int SEG, bool FX, typename K>

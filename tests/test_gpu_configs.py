@@ -4,10 +4,11 @@
   64 channels 700-800 MHz; seeds 4000-4095; Omega_m in [0.25, 0.35], sigma_8 in [0.7, 0.9] per member)
   in one captured graph.  With the deterministic scatter every member's maps must equal those of its own
   single run bit for bit, on every replay; sigma^2 likewise.
-- Learned C_l emulator (the mean of 4 networks): trained on 512 Latin-hypercube cosmologies x 8
-  realisations of 256^3 / Nside 256 / 64-channel mocks, judged on 64 held-out cosmologies x 16 fresh
-  realisations, against the method without it (the nearest training cosmology).  Mocks and training are
-  deterministic, so the error is the same number on every run.
+- Learned C_l emulator (one network): trained on 512 Latin-hypercube cosmologies x 8 realisations of
+  256^3 / Nside 256 / 64-channel mocks, judged on 64 held-out cosmologies x 16 fresh realisations,
+  against the method without it (the nearest training cosmology).  Mocks and training are deterministic,
+  so the error is the same number on every run; at this size networks from other seeds land within
+  0.006 sigma of it (the small test's noisier spectra need the mean of 4).
 """
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -64,7 +65,7 @@ def test_config5_emulator():
     print(f"emulator: {512 * 8} training mocks in {time.perf_counter() - t0:.1f} s", flush=True)
     cls_te, _ = em.simulate(test_c, 16, spec, seed0=900000)
     print(f"emulator: {64 * 16} held-out mocks done, {t_mock * 1e3:.2f} ms per mock", flush=True)
-    emu, losses = em.train_emulator(train_c, cls_tr, spec, steps=20000, batch=4096, lr0=2e-3, seed=0, n_nets=4)
+    emu, losses = em.train_emulator(train_c, cls_tr, spec, steps=20000, batch=4096, lr0=2e-3, seed=0)
     res = em.evaluate(emu, train_c, cls_tr, test_c, cls_te, spec)
     xq = em.features(em.latin_hypercube(4096, seed=3), spec)
     emu.predict(xq[:4096])
@@ -78,5 +79,5 @@ def test_config5_emulator():
           f"{t_mock / t_eval:.0f}x the mock rate", flush=True)
     assert losses[-200:].mean() < 0.05 * losses[:20].mean()
     assert e["rms_debiased"] < 0.5 * n["rms_debiased"]
-    assert e["rms_debiased"] < 0.2  # the target (measured 0.177 for the mean of 4 networks)
-    assert t_mock / t_eval > 100     # 4 networks' spectra of a cosmology against one 256^3 mock (measured 280x)
+    assert e["rms_debiased"] < 0.2  # the target (measured 0.192; networks of seeds 1-3: 0.196-0.198)
+    assert t_mock / t_eval > 1000   # the target (measured 1264x)
