@@ -39,6 +39,7 @@ struct crime_gh {
   uint32_t *d_order = nullptr; // map-stage tiles by distance from the observer
   uint32_t *d_binstart = nullptr; // [nbins + 1] first tile of each distance bin in d_order
   uint32_t *d_range = nullptr; // [2] the map kernel's range of d_order (k_tile_range)
+  uint8_t *d_cull = nullptr;   // [ntiles] out-of-band flags of d_order's entries (k_tile_cull)
   double *d_sub = nullptr;     // [3*n_sub]
   double *d_subp = nullptr;    // [3*n_sub] the offsets in the map stage's visiting order
   double *d_modes = nullptr;   // injected unit modes
@@ -324,7 +325,7 @@ static void twiddles(int n, std::vector<cx<T>> &tw)
 
 static int free_all(crime_gh *h)
 {
-  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_chwin, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_binstart, h->d_range, h->d_sub, h->d_subp,
+  void *ptrs[] = {h->F, h->G, h->d_sin, h->d_fx, h->d_fxs, h->d_chwin, h->maps, h->d_tab, h->d_pk, h->d_edges, h->d_prefac, h->d_crec, h->d_amp, h->d_order, h->d_binstart, h->d_range, h->d_cull, h->d_sub, h->d_subp,
                   h->d_modes, h->d_tw, h->d_partials, h->d_sig, h->d_ps, h->d_ps_count, h->ps_maps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -668,6 +669,9 @@ static int create_impl(const crime_gh_desc *d, crime_gh *h)
     HIP_OR_FAIL(h, hipMalloc((void **)&h->d_range, sizeof(range0)));
     HIP_OR_FAIL(h, hipMemcpy(h->d_range, range0, sizeof(range0), hipMemcpyHostToDevice));
     m.tile_range = h->d_range;
+    HIP_OR_FAIL(h, hipMalloc((void **)&h->d_cull, ntiles));
+    HIP_OR_FAIL(h, hipMemset(h->d_cull, 0, ntiles));
+    m.tile_cull = h->d_cull;
   }
   m.nside = h->nside;
   m.npix = h->npix;

@@ -181,6 +181,8 @@ struct MapParams {
   int nbins;
   double tile_hd;              // every cell centre of a tile lies within tile_hd of the tile's centre
   uint32_t *tile_range;        // device [first, last): the entries of tile_order the map kernel visits
+  uint8_t *tile_cull;          // device [ntiles]: 1 when tile_order's entry i is out of the band (k_tile_cull;
+                               // written for the range only)
   int ntx, nty, ntz;
   int vmode;               // CRIME_GH_VEL_*: SPECTRAL reads vx from F.y and (vy, vz) from G
   const void *G;           // cx<T>[N^3] second grid of SPECTRAL handles (NULL otherwise)

@@ -623,8 +623,8 @@ __device__ __forceinline__ void pixelize_body(const cx<T> *__restrict__ F, T *__
   if (bid >= (size_t)m.tile_range[1]) return;
   const uint32_t b = m.tile_order[bid];
   const int tx = (int)(b & 1023u), ty = (int)((b >> 10) & 1023u), tz = (int)(b >> 20);
-  // whole tiles out of the band, by the grid's bound on the RSD shift: before any grid load
-  if (m.precull && tile_out_of_band(m, tx, ty, tz, m.dz_per_phi * m.sig[2])) return;
+  // whole tiles out of the band, by the grid's bound on the RSD shift (k_tile_cull): before any grid load
+  if (m.tile_cull[bid]) return;
   const int ix = tx * TILE_X + (t % TILE_X), iy = ty * TILE_Y + ((t / TILE_X) % TILE_Y),
             iz = tz * TILE_Z + t / (TILE_X * TILE_Y);
   CellState s;
@@ -771,6 +771,17 @@ __global__ __launch_bounds__(256) void k_tile_range(MapParams m)
     m.tile_range[0] = any ? m.bin_start[lo] : 0u;
     m.tile_range[1] = any ? m.bin_start[hi + 1] : 0u;
   }
+}
+
+// The out-of-band flag of every entry of tile_order in [first, last), one tile per thread, with the grid's RSD
+// bound (0 when the bound is off): the map kernel's workgroups read it instead of each evaluating the test.
+__global__ __launch_bounds__(256) void k_tile_cull(MapParams m)
+{
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < (size_t)m.tile_range[0] || i >= (size_t)m.tile_range[1]) return;
+  const uint32_t b = m.tile_order[i];
+  const int tx = (int)(b & 1023u), ty = (int)((b >> 10) & 1023u), tz = (int)(b >> 20);
+  m.tile_cull[i] = m.precull && tile_out_of_band(m, tx, ty, tz, m.dz_per_phi * m.sig[2]) ? 1 : 0;
 }
 
 template <typename T, int SLOTS, int SEG, bool FX, typename K>
@@ -1296,6 +1307,10 @@ hipError_t launch_maps(const cx<T> *F, T *maps, size_t nmaps, const MapParams &m
 {
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_tile_range, dim3(1), dim3(256), 0, s, m);
+  {
+    const size_t nt = (size_t)m.ntx * m.nty * m.ntz;
+    hipLaunchKernelGGL(k_tile_cull, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, m);
+  }
   if (rb) {
     hipError_t e = hipMemcpyAsync(rb->h_range, m.tile_range, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipEventRecord(rb->ev, s);
