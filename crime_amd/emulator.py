@@ -244,12 +244,18 @@ class Emulator:
         self.mlps = list(mlps) if isinstance(mlps, (list, tuple)) else [mlps]
         self.mlp = self.mlps[0]
         self.scale, self.dnu = scale, dnu
+        self._z = None   # float32 input buffer, reused: a fresh one costs its page faults on every call
         for mlp in self.mlps:
             mlp.set_params(fold_standardisation(mlp.params(), x_mean, x_std, y_mean, y_std))
 
     def predict_y(self, x):
-        z = np.array(x, dtype=np.float32, copy=True)
-        z[:, 6] = (z[:, 6] - z[:, 5]) * np.float32(1.0 / self.dnu)
+        x = np.asarray(x).reshape(-1, NIN)
+        if self._z is None or self._z.shape[0] < x.shape[0]:
+            self._z = np.empty((x.shape[0], NIN), np.float32)
+        z = self._z[:x.shape[0]]
+        np.copyto(z, x, casting="unsafe")
+        z[:, 6] -= z[:, 5]
+        z[:, 6] *= np.float32(1.0 / self.dnu)
         return Mlp.forward_mean(self.mlps, z) if len(self.mlps) > 1 else self.mlps[0].forward(z)
 
     def close(self):
@@ -257,8 +263,11 @@ class Emulator:
             mlp.close()
 
     def predict(self, x):
-        """Binned C_l at raw inputs x (n, 8)."""
-        return np.sinh(self.predict_y(x)) * np.float32(self.scale)
+        """Binned C_l at raw inputs x (n, 8).  Calls on one Emulator must not overlap (a shared input buffer)."""
+        y = self.predict_y(x)
+        np.sinh(y, out=y)
+        y *= np.float32(self.scale)
+        return y
 
 
 def gaussian_sigma(mean, spec, nseeds):
