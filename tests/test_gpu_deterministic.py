@@ -98,3 +98,23 @@ def test_tile_culling_is_exact(cfgdir, prec):
     print(f"tile culling ({prec}): {np.count_nonzero(full)} occupied bins", flush=True)
     assert np.count_nonzero(full) > 1000
     assert np.array_equal(culled, full)
+
+
+@pytest.mark.parametrize("det", [0, 1])
+def test_band_out_of_reach(cfgdir, det):
+    """A band that no cell of the box can reach (540-560 MHz, z > 1.5, in a 1500 Mpc/h box): the tile
+    range is empty, the map kernel launches no workgroup (handle stream) or only ones that return at once
+    (captured graph), and the maps are zero both ways; a band inside the box afterwards still fills."""
+    far = small_params(cfgdir, f"far_{det}", n_grid=64, n_side=16, precision="single", l_box=1500.0,
+                       nu_lo=540.0, nu_hi=560.0, extra=dict(deterministic=det))
+    with crime_amd.GetHI(far) as gh:
+        gh.run()
+        assert not np.any(gh.maps(np.float64))
+        with crime_amd.Batch([gh], n_streams=1) as b:
+            b.run()
+        assert not np.any(gh.maps(np.float64))
+    near = small_params(cfgdir, f"near_{det}", n_grid=64, n_side=16, precision="single", l_box=1500.0,
+                        nu_lo=1250.0, nu_hi=1300.0, extra=dict(deterministic=det))
+    with crime_amd.GetHI(near) as gh:
+        gh.run()
+        assert np.count_nonzero(gh.maps(np.float64)) > 100
