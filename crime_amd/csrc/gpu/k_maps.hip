@@ -402,8 +402,9 @@ __device__ __forceinline__ int screen_pixel(const MapParams &m, const SampleS &q
     const int jp = (twoK >> 1) + (int)f1, jm = (twoK >> 1) + (int)f2;
     const int ir = nside + 1 + jp - jm;
     const int kshift = 1 - (ir & 1);
-    // (jp + jm - Nside + kshift + 1) / 2 in [-Nside/2, 5 Nside + 1]: halved with an even offset that makes it
-    // non-negative
+    // (jp + jm - Nside + kshift + 1) / 2: jp + jm > 2 N (1/2 + tt) - 2 >= N - 2, so the numerator is at least 0
+    // and the floor below is C's truncating division; the even offset keeps the shift a floor also for the
+    // values of samples that miss a margin (discarded)
     int ip = (int)((unsigned)(jp + jm - nside + kshift + 1 + 8 * nside) >> 1) - 4 * nside;
     ip = ip < 0 ? ip + 4 * nside : (ip >= 4 * nside ? ip - 4 * nside : ip);
     pix = nside * (nside - 1) * 2 + (int)__umul24((unsigned)(ir - 1), (unsigned)(4 * nside)) + ip;
