@@ -94,12 +94,12 @@ __global__ __launch_bounds__(256) void k_amp_table(FillParams f, AmpEntry<T> *__
 
 // F = delta + i phi_scale phi at full-grid position (kx, ky, kz), kx in [0, N) (N a compile-time
 // constant: the mirror indices are masks or constant divisions, not runtime integer divisions)
+// (A, A psi) of mode (kx, ky, kz): a function of |k|^2 only, so kz and N - kz share it
 template <typename T, int N>
-__device__ __forceinline__ cx<T> fill_element(const FillParams &f, const AmpEntry<T> *__restrict__ amp, int kx,
-                                              int ky, int kz)
+__device__ __forceinline__ cx<T> mode_amp(const FillParams &f, const AmpEntry<T> *__restrict__ amp, int kx, int ky,
+                                          int kz)
 {
   constexpr int n = N, h = N / 2;
-  if ((kx | ky | kz) == 0) return {(T)0, (T)0};
   // |k| components with the reference's +Nyquist convention (fourier.c:264-283); only squares matter
   int ax = kx <= h ? kx : n - kx;
   int ay = ky <= h ? ky : n - ky;
@@ -113,6 +113,15 @@ __device__ __forceinline__ cx<T> fill_element(const FillParams &f, const AmpEntr
     float a = amp[m2];
     ab = {a, a * ((float)f.psi_c * __builtin_amdgcn_rcpf((float)m2))};
   }
+  return ab;
+}
+
+// the element from its amplitude pair ab = mode_amp(kx, ky, kz)
+template <typename T, int N>
+__device__ __forceinline__ cx<T> fill_element_a(const FillParams &f, cx<T> ab, int kx, int ky, int kz)
+{
+  constexpr int n = N, h = N / 2;
+  if ((kx | ky | kz) == 0) return {(T)0, (T)0};
   int myn = (n - ky) % n, mzn = (n - kz) % n;  // -k in y and z
   cx<T> g;
   if (kx > 0 && kx < h) {
@@ -162,7 +171,13 @@ __global__ __launch_bounds__((FillCfg<T, N, V>::NT)) void k_fill_z(cx<T> *__rest
   cx<T> *col = F + (size_t)kx + (size_t)N * ky;
   // the generator is too large to unroll into the register tile: stage the modes in LDS first
   const AmpEntry<T> *amp = (const AmpEntry<T> *)f.amp;
-  for (int e = t; e < N; e += C::TC) lds[e * C::TX + c] = fill_element<T, N>(f, amp, kx, ky, e);
+  // kz = e and N - e in one step: one read of their shared amplitude (a scattered gather, a cache line per
+  // lane) for two elements
+  for (int e = t; e <= N / 2; e += C::TC) {
+    const cx<T> ab = mode_amp<T, N>(f, amp, kx, ky, e);
+    lds[e * C::TX + c] = fill_element_a<T, N>(f, ab, kx, ky, e);
+    if (e > 0 && e < N / 2) lds[(N - e) * C::TX + c] = fill_element_a<T, N>(f, ab, kx, ky, N - e);
+  }
   __syncthreads();
   auto load = [&](int e) { return lds[e * C::TX + c]; };
   auto store = [&](int e, cx<T> v) { col[(size_t)e * n2] = v; };
